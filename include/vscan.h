@@ -1,0 +1,184 @@
+/*
+ * vscan.h — C ABI of the MI355X-native per-frame depth-fusion pipeline.
+ *
+ * This is the drop-in boundary for the scanner's RGB-D GPU engine: the four
+ * operator classes and three containers that the reference drives once per
+ * frame (reference: VCL_Volume_Scanner/STAC/Engine/GPU/_yooji_cuda_img_proc.h:2-29,
+ * _yooji_cuda_pose_tracker.h:2-42, _yooji_cuda_volume_integrator.h:2-52,
+ * _yooji_cuda_tsdf_renderer.h:2-96; containers STAC/Object/GPU/_yooji_cuda_object_cube.h:9-101,
+ * _yooji_cuda_tracking_state.h:4-168, _yooji_cuda_rgbd_frame.h:4-91).
+ *
+ * Differences from the reference boundary, all deliberate:
+ *   - opaque handles instead of C++ classes; no process-global state (the
+ *     reference keeps camera/volume parameters in per-translation-unit
+ *     __constant__ singletons, so it can run one session per process);
+ *   - every call takes an explicit stream (a hipStream_t passed as void*;
+ *     NULL selects the session's own stream) and is asynchronous unless it
+ *     says otherwise;
+ *   - status codes instead of bool;
+ *   - the Gauss-Newton loop of the tracker runs on the device: no host round
+ *     trip per iteration, the pose stays resident in HBM (FP64 master copy).
+ *
+ * Conventions (same as the reference):
+ *   - depth input is uint16 millimetres, converted to metres as mm * 0.001f
+ *     (STAC/Util/z_yooji_frame_capture.cpp:597,626); 0 = invalid;
+ *   - T_gc maps world ("global") points into the camera frame, row-major 4x4;
+ *     T_cg is its inverse;
+ *   - voxel (i,j,k) has its centre at origin + (i+0.5)*voxel; the linear index
+ *     used by vs_volume_download/upload is k*W*H + j*W + i (x fastest);
+ *   - invalid vertex/normal = (-100,-100,-100), invalid depth = 0.
+ */
+#ifndef VSCAN_H
+#define VSCAN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vs_ctx vs_ctx;
+typedef struct vs_session vs_session;
+typedef void *vs_stream; /* hipStream_t */
+
+typedef enum {
+    VS_OK = 0,
+    VS_EINVAL = 1,          /* bad argument or descriptor */
+    VS_ENOMEM = 2,          /* device allocation failed */
+    VS_EHIP = 3,            /* a HIP runtime call failed */
+    VS_TRACK_LOST = 4,      /* Cholesky pivot below 1e-8 (or non-finite): pose kept, frame not fused */
+    VS_MOTION_REJECTED = 5, /* optional motion gate fired: pose kept, frame not fused */
+    VS_BUDGET = 6,          /* HBM budget exhausted (serving admission) */
+    VS_TOO_FEW_POINTS = 7   /* optional minimum-correspondence gate fired */
+} vs_status;
+
+typedef enum { VS_TSDF_F32 = 0, VS_TSDF_F16 = 1 } vs_tsdf_type;
+typedef enum { VS_WEIGHT_F32 = 0, VS_WEIGHT_U8 = 1 } vs_weight_type;
+
+/* One scan session: camera, pyramid, tracker, volume and filter settings.
+ * Replaces CKvYooji_Scanner_Parameter (STAC/Object/CPU/_yooji_scanner_parameter.h:4-71)
+ * plus the per-operator ip_Initialize_Parameters calls
+ * (z_yooji_cuda_pose_tracker.cu:735-785, z_yooji_cuda_volume_integrator.cu:822-869,
+ *  z_yooji_cuda_tsdf_renderer.cu:1577-1639) and the bilateral globals
+ * (z_yooji_cuda_img_proc.cu:42-45). */
+typedef struct {
+    int width, height;           /* level-0 depth frame size (pixels) */
+    float fx, fy, cx, cy;        /* level-0 intrinsics; level l uses fx/2^l, c <- (c-0.5)/2 per level */
+    int levels;                  /* pyramid levels, 1..3 */
+    int iters[3];                /* Gauss-Newton iterations per level, index 0 = finest */
+    float icp_dist2;             /* correspondence gate on squared point distance (m^2) */
+    int icp_min_valid;           /* minimum valid correspondences per solve (0 = no gate, as the GPU reference) */
+    float max_motion_m;          /* optional motion gate: translation limit (m), <= 0 disables */
+    float max_motion_deg;        /* optional motion gate: rotation limit (deg), <= 0 disables */
+    int vol_dim[3];              /* voxels per axis; each a multiple of 8 */
+    float origin[3];             /* world position of the volume's min corner (m) */
+    float voxel;                 /* voxel edge length (m) */
+    float mu;                    /* truncation distance (m) */
+    float max_weight;            /* running-average weight cap */
+    int tsdf_type;               /* vs_tsdf_type */
+    int weight_type;             /* vs_weight_type */
+    float bil_sigma_s;           /* bilateral spatial sigma (pixels) */
+    float bil_sigma_r;           /* bilateral range sigma (m) */
+    int bil_radius;              /* bilateral half window (1..3) */
+} vs_session_desc;
+
+/* Fill *d with the reference's defaults for a W x H camera and an N^3 cube of
+ * edge `cube_m` metres: 3 levels, iterations {10,7,7}, mu = 5*voxel... Callers
+ * normally overwrite the fields their workload names. */
+void vs_session_desc_default(vs_session_desc *d, int width, int height, int n, float cube_m);
+
+/* ---- context ---------------------------------------------------------- */
+vs_status vs_ctx_create(int device, vs_ctx **out);
+vs_status vs_ctx_destroy(vs_ctx *ctx);
+const char *vs_status_string(int status);
+const char *vs_last_error(void); /* thread-local text of the last failure */
+
+/* ---- sessions (GKvObjectCube + GKvTrackingState + GKvRgbdFrame) -------- */
+/* Device bytes one session holds (volume, masks, frame and model buffers). */
+size_t vs_session_bytes(const vs_session_desc *desc);
+/* Allocate a session with TSDF = 1, weight = 0 (z_yooji_object_scanner.cpp:343-347)
+ * and pose T_gc0 (NULL = identity). */
+vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const double T_gc0[16], vs_session **out);
+vs_status vs_session_destroy(vs_session *s);
+/* Clear the volume and frame counter and set the pose again (stream-ordered). */
+vs_status vs_session_reset(vs_session *s, const double T_gc0[16], vs_stream stream);
+vs_status vs_session_desc_get(const vs_session *s, vs_session_desc *out);
+/* Block until all work queued on the session's own stream has finished. */
+vs_status vs_session_sync(vs_session *s);
+
+/* ---- per-stage operators ---------------------------------------------- */
+/* Copy one uint16 mm frame into the session's raw-frame buffer. */
+vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream);
+/* A2-A3: bilateral filter of the raw frame into pyramid level 0 (metres).
+ * Replaces LGKvImageProcessor::bfd_Bilateral_Filter_Depth (z_yooji_cuda_img_proc.cu:394-417). */
+vs_status vs_bilateral(vs_session *s, vs_stream stream);
+/* A4: levels 1..L-1 from level 0 (5x5 binomial blur + decimation), as
+ * cpdd_Construct_Pyramids_Down_Depth (z_yooji_image_processor.cpp:368-407). */
+vs_status vs_pyramid(vs_session *s, vs_stream stream);
+/* A6-A7: vertex and normal maps of the frame at `level` into device buffers
+ * of W_l*H_l*3 floats each (cpm_Compute_Point_Map / enm_Estimate_Normal_Map,
+ * z_yooji_image_processor.cpp:119-227). Not used by vs_fuse_frame, which
+ * back-projects inline like the reference GPU tracker. */
+vs_status vs_vertex_normal_maps(vs_session *s, int level, float *vtx_dev, float *nrm_dev, vs_stream stream);
+/* A8-A12: projective point-to-plane ICP of the frame pyramid against the
+ * model maps of the previous raycast; updates the resident pose.
+ * Replaces LGKvPoseTracker::tp_Track_Pose (z_yooji_cuda_pose_tracker.cu:790-1042).
+ * T_01_init (nullable, row-major, maps camera t0 into camera t1) forces a
+ * single level-0 pass, as the reference's in_pose_init_t0_t1. The status of
+ * the solve is reported by vs_get_frame_status after the stream completes. */
+vs_status vs_track(vs_session *s, const double *T_01_init, vs_stream stream);
+/* A13-A15: brick visibility + TSDF update with the raw frame at the current pose.
+ * Replaces LGKvVolumeIntegrator::cdtoc_Convert_Depth_to_TSDF_on_Cube
+ * (z_yooji_cuda_volume_integrator.cu:936-972) with the per-frame brick-mask
+ * clear of cdtocc (:1006-1010). */
+vs_status vs_integrate(vs_session *s, vs_stream stream);
+/* A16-A18: raycast the volume at `level` into the session's model maps.
+ * Replaces LGKvRendererTSDF::rmi_Render_Maps_for_ICP (z_yooji_cuda_tsdf_renderer.cu:1642-1700). */
+vs_status vs_raycast(vs_session *s, int level, vs_stream stream);
+/* A19: one fused frame (filter -> pyramid -> track (frames > 0) -> integrate
+ * raw depth -> raycast every level), as so_Scan_Object
+ * (z_yooji_object_scanner.cpp:472-629). Asynchronous; see vs_get_frame_status. */
+vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream);
+/* Many sessions, one frame each, in one batched launch per stage. Sessions
+ * must share tsdf/weight types. depths[i] are device pointers unless is_host. */
+vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t *const *depths,
+                        int n, int is_host, vs_stream stream);
+
+/* ---- state access (synchronous unless noted) --------------------------- */
+/* Status of the last tracked frame (vs_status value) and frames fused so far. */
+vs_status vs_get_frame_status(vs_session *s, int *last_status, int *frames_fused, int *icp_iters);
+vs_status vs_get_pose(vs_session *s, double T_gc[16]);
+vs_status vs_set_pose(vs_session *s, const double T_gc[16], vs_stream stream);
+/* Pyramid level `level` of the filtered frame (W_l*H_l floats, metres). */
+vs_status vs_get_filtered(vs_session *s, int level, float *host);
+vs_status vs_set_filtered(vs_session *s, int level, const float *host);
+/* Model maps written by vs_raycast: depth W_l*H_l, vertex/normal W_l*H_l*3
+ * (AoS, camera frame), shaded normal image W_l*H_l. Any pointer may be NULL. */
+vs_status vs_get_maps(vs_session *s, int level, float *depth, float *vtx, float *nrm, uint8_t *img);
+vs_status vs_set_maps(vs_session *s, int level, const float *vtx, const float *nrm);
+/* Whole volume in the reference's linear layout; tsdf/weight converted to
+ * float. brick_mask: (W/8)(H/8)(D/8) bytes of the last frame's visibility. */
+vs_status vs_volume_download(vs_session *s, float *tsdf, float *weight, uint8_t *brick_mask);
+vs_status vs_volume_upload(vs_session *s, const float *tsdf, const float *weight);
+/* Normal equations of one ICP iteration at `level` for the float pose
+ * T_21 (camera t1 -> camera t0): 36 + 6 doubles and the valid count, plus an
+ * optional W_l*H_l validity mask. Diagnostic and parity entry point. */
+vs_status vs_icp_system(vs_session *s, int level, const float T_21[16], double ATA[36], double ATb[6],
+                        long long *valid, uint8_t *mask_host);
+/* Per-stage kernel times of the last vs_fuse_frame on this session (ms):
+ * [bilateral, pyramid, track, integrate, raycast]. Valid after sync when
+ * timing was enabled with vs_session_set_timing. */
+vs_status vs_session_set_timing(vs_session *s, int enable);
+vs_status vs_get_stage_ms(vs_session *s, float ms[5]);
+/* Counts from the last integrate: visible bricks and updated voxels. */
+vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long long *updated_voxels);
+/* Tap counts of one instrumented raycast at `level` with the current pose and
+ * volume: nearest taps, trilinear evaluations, hits (synchronous). They price
+ * the raycast's algorithmic bytes; no counterpart in the reference. */
+vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VSCAN_H */

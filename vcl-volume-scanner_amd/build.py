@@ -1,0 +1,67 @@
+"""Build recipe for the fusion library (in-tree, for gfx950).
+
+    python vcl-volume-scanner_amd/build.py
+
+produces, next to this file,
+  libvscan.so        HIP kernels + the C ABI of include/vscan.h (hipcc, gfx950)
+  libvscan_scene.so  the synthetic depth-stream generator of include/vscan_scene.h (g++)
+Objects go to _build/. Every HIP source is compiled with -ffp-contract=off
+(index and mask arithmetic must round once per operation) and with packed
+FP32 instructions disabled.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("VS_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["vs_img.hip", "vs_icp.hip", "vs_vol.hip", "vs_api.hip"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    f"-I{INCLUDE}", f"-I{CSRC}",
+]
+
+LIB = os.path.join(HERE, "libvscan.so")
+SCENE_LIB = os.path.join(HERE, "libvscan_scene.so")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
+        for r in ex.map(_run, cmds):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout + r.stderr)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    tmp = SCENE_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", f"-I{INCLUDE}",
+          os.path.join(CSRC, "scene.cpp"), "-o", tmp])
+    os.replace(tmp, SCENE_LIB)
+    return LIB, SCENE_LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

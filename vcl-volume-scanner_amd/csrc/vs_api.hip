@@ -1,0 +1,859 @@
+// vs_api.hip — host side of the C ABI declared in include/vscan.h: contexts,
+// sessions (one HBM allocation each), batches and the per-frame launch
+// sequence, optionally replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "vs_device.hpp"
+#include "vs_launch.hpp"
+#include "vscan.h"
+
+namespace vs {
+void launch_track_setup_forced(const SessParams *P, int nsess, const double *T_init, hipStream_t s);
+void launch_pose_derived(const SessParams *P, hipStream_t s);
+void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
+                          unsigned long long *stats, hipStream_t s);
+void launch_fuse_ok(const SessParams *P, int nsess, hipStream_t s);
+}  // namespace vs
+
+using namespace vs;
+
+namespace {
+
+thread_local std::string g_err;
+
+vs_status fail(vs_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(VS_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+struct Layout {
+    size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
+    size_t tsdf, weight, bvis, bocc, icp_part, diag_part, diag_mask, blist, bcount, params, work[kMaxLevels],
+        stats;
+    size_t total;
+    size_t nvox;
+    uint32_t nbricks, nwords;
+    int tiles[kMaxLevels];
+};
+
+int tiles_of(int W, int H) { return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH); }
+
+Layout make_layout(const vs_session_desc &d) {
+    Layout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align_up(off + bytes);
+        return o;
+    };
+    const int W = d.width, H = d.height;
+    L.nvox = (size_t)d.vol_dim[0] * d.vol_dim[1] * d.vol_dim[2];
+    L.nbricks = (uint32_t)((d.vol_dim[0] / 8) * (d.vol_dim[1] / 8) * (d.vol_dim[2] / 8));
+    L.nwords = (L.nbricks + 31) / 32;
+    const size_t sv = d.tsdf_type == VS_TSDF_F16 ? 2 : 4, sw = d.weight_type == VS_WEIGHT_U8 ? 1 : 4;
+    L.state = take(sizeof(SessState));
+    L.params = take(sizeof(SessParams));
+    L.raw = take((size_t)W * H * 2);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        const int Wl = W >> l, Hl = H >> l;
+        const size_t n = (size_t)Wl * Hl;
+        L.tiles[l] = l < d.levels ? tiles_of(Wl, Hl) : 0;
+        L.filt[l] = take(n * 4);
+        L.mvtx[l] = take(n * 16);
+        L.mnrm[l] = take(n * 16);
+        L.mimg[l] = take(n);
+        L.work[l] = take((size_t)std::max(L.tiles[l], 1) * sizeof(WorkItem));
+    }
+    L.tsdf = take(L.nvox * sv);
+    L.weight = take(L.nvox * sw);
+    L.bvis = take((size_t)L.nwords * 4);
+    L.bocc = take((size_t)L.nwords * 4);
+    L.icp_part = take((size_t)L.tiles[0] * kIcpVals * 8);
+    L.diag_part = take((size_t)L.tiles[0] * kIcpVals * 8);
+    L.diag_mask = take((size_t)W * H);
+    L.blist = take((size_t)L.nbricks * sizeof(int2));
+    L.bcount = take(16);
+    L.stats = take(64);
+    L.total = off;
+    return L;
+}
+
+vs_status check_desc(const vs_session_desc *d) {
+    if (!d) return fail(VS_EINVAL, "null descriptor");
+    if (d->width < 16 || d->height < 16 || d->width > 8192 || d->height > 8192)
+        return fail(VS_EINVAL, "frame size %dx%d out of range", d->width, d->height);
+    if (d->levels < 1 || d->levels > kMaxLevels) return fail(VS_EINVAL, "levels must be 1..3");
+    for (int l = 0; l < d->levels; ++l)
+        if (d->iters[l] < 0 || d->iters[l] > 100) return fail(VS_EINVAL, "iters out of range");
+    for (int k = 0; k < 3; ++k)
+        if (d->vol_dim[k] < 8 || d->vol_dim[k] % 8 || d->vol_dim[k] > 2048)
+            return fail(VS_EINVAL, "vol_dim[%d]=%d must be a multiple of 8 in [8, 2048]", k, d->vol_dim[k]);
+    if (!(d->voxel > 0) || !(d->mu > 0) || !(d->max_weight >= 1)) return fail(VS_EINVAL, "voxel/mu/max_weight");
+    if (d->weight_type == VS_WEIGHT_U8 && d->max_weight > 255) return fail(VS_EINVAL, "u8 weight cap > 255");
+    if (d->tsdf_type < 0 || d->tsdf_type > 1 || d->weight_type < 0 || d->weight_type > 1)
+        return fail(VS_EINVAL, "storage type");
+    if (d->bil_radius < 1 || d->bil_radius > 3 || !(d->bil_sigma_s > 0) || !(d->bil_sigma_r > 0))
+        return fail(VS_EINVAL, "bilateral parameters");
+    if (!(d->fx > 0) || !(d->fy > 0)) return fail(VS_EINVAL, "focal length");
+    return VS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context, batches
+// ---------------------------------------------------------------------------
+struct Batch {
+    int n = 0, tt = 0, wt = 0, levels = 0;
+    int max_iters[kMaxLevels] = {0, 0, 0};
+    SessParams *dP = nullptr;  // device array
+    WorkItem *dWork[kMaxLevels] = {nullptr, nullptr, nullptr};
+    int nWork[kMaxLevels] = {0, 0, 0};
+    BrickList bl{};
+    int integ_grid = 1;
+    void *owned = nullptr;  // allocation owned by a context batch (null for a session's own)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<vs_session *> members;
+};
+
+struct vs_ctx {
+    int device = 0;
+    std::map<std::vector<vs_session *>, std::unique_ptr<Batch>> batches;
+    std::vector<vs_session *> sessions;
+    bool use_graph = true;
+};
+
+struct vs_session {
+    vs_ctx *ctx = nullptr;
+    vs_session_desc d{};
+    Layout L{};
+    char *mem = nullptr;
+    SessParams hp{};
+    SessState *st = nullptr;
+    hipStream_t stream = nullptr;
+    hipStream_t last = nullptr;
+    Batch self;
+    bool timing = false;
+    hipEvent_t ev[6] = {};
+    float stage_ms[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+void free_batch_graph(Batch &b) {
+    if (b.gexec) hipGraphExecDestroy(b.gexec);
+    if (b.graph) hipGraphDestroy(b.graph);
+    b.gexec = nullptr;
+    b.graph = nullptr;
+}
+
+void drop_batches_with(vs_ctx *ctx, vs_session *s) {
+    for (auto it = ctx->batches.begin(); it != ctx->batches.end();) {
+        if (std::find(it->first.begin(), it->first.end(), s) != it->first.end()) {
+            free_batch_graph(*it->second);
+            if (it->second->owned) hipFree(it->second->owned);
+            it = ctx->batches.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+hipStream_t pick(vs_session *s, vs_stream st) {
+    hipStream_t h = st ? (hipStream_t)st : s->stream;
+    s->last = h;
+    return h;
+}
+
+// The per-frame launch sequence of a batch (so_Scan_Object order).
+void enqueue_step(const Batch &B, hipStream_t s, vs_session *timed) {
+    auto mark = [&](int i) {
+        if (timed) hipEventRecord(timed->ev[i], s);
+    };
+    mark(0);
+    launch_track_setup(B.dP, B.n, nullptr, s);
+    launch_bilateral(B.dP, B.dWork[0], B.nWork[0], s);
+    mark(1);
+    for (int l = 1; l < B.levels; ++l) launch_pyr_down(B.dP, B.dWork[l], B.nWork[l], l, s);
+    mark(2);
+    for (int l = B.levels - 1; l >= 0; --l)
+        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], l, it, s);
+    launch_track_finalize(B.dP, B.n, s);
+    mark(3);
+    launch_vis_clear(B.dP, B.n, B.bl, s);
+    launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, s);
+    launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, s);
+    mark(4);
+    for (int l = 0; l < B.levels; ++l) launch_raycast(B.dP, B.dWork[l], B.nWork[l], l, B.tt, B.wt, s);
+    mark(5);
+}
+
+vs_status run_step(vs_ctx *ctx, Batch &B, hipStream_t s, vs_session *timed) {
+    if (timed || !ctx->use_graph) {
+        enqueue_step(B, s, timed);
+        HIP_TRY(hipGetLastError());
+        return VS_OK;
+    }
+    if (!B.gexec) {
+        hipStream_t cap;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        enqueue_step(B, cap, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        hipStreamDestroy(cap);
+        if (e != hipSuccess) return fail(VS_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+        B.graph = g;
+        HIP_TRY(hipGraphInstantiate(&B.gexec, g, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(B.gexec, s));
+    return VS_OK;
+}
+
+vs_status sync_last(vs_session *s) {
+    HIP_TRY(hipStreamSynchronize(s->last ? s->last : s->stream));
+    if (s->last != s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
+    return VS_OK;
+}
+
+vs_status copy_depth(vs_session *s, const uint16_t *depth, int is_host, hipStream_t st) {
+    if (!depth) return fail(VS_EINVAL, "null depth");
+    const size_t bytes = (size_t)s->d.width * s->d.height * 2;
+    HIP_TRY(hipMemcpyAsync((void *)s->hp.raw, depth, bytes, is_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                           st));
+    return VS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vs_status_string(int st) {
+    switch (st) {
+    case VS_OK: return "ok";
+    case VS_EINVAL: return "invalid argument";
+    case VS_ENOMEM: return "out of device memory";
+    case VS_EHIP: return "HIP runtime error";
+    case VS_TRACK_LOST: return "tracking lost (singular system)";
+    case VS_MOTION_REJECTED: return "motion rejected";
+    case VS_BUDGET: return "HBM budget exhausted";
+    case VS_TOO_FEW_POINTS: return "too few correspondences";
+    default: return "unknown status";
+    }
+}
+const char *vs_last_error(void) { return g_err.c_str(); }
+
+void vs_session_desc_default(vs_session_desc *d, int width, int height, int n, float cube_m) {
+    std::memset(d, 0, sizeof(*d));
+    d->width = width;
+    d->height = height;
+    d->fx = d->fy = 525.0f * width / 640.0f;
+    d->cx = width * 0.5f - 0.5f;
+    d->cy = height * 0.5f - 0.5f;
+    d->levels = 3;
+    d->iters[0] = 10;
+    d->iters[1] = 7;
+    d->iters[2] = 7;
+    d->icp_dist2 = (0.055f * cube_m) * (0.055f * cube_m);
+    d->vol_dim[0] = d->vol_dim[1] = d->vol_dim[2] = n;
+    d->origin[0] = d->origin[1] = d->origin[2] = -0.5f * cube_m;
+    d->voxel = cube_m / n;
+    d->mu = 5.0f * cube_m / n;
+    d->max_weight = 100.0f;
+    d->tsdf_type = VS_TSDF_F32;
+    d->weight_type = VS_WEIGHT_F32;
+    d->bil_sigma_s = 4.0f;
+    d->bil_sigma_r = 0.1f;
+    d->bil_radius = 2;
+}
+
+vs_status vs_ctx_create(int device, vs_ctx **out) {
+    if (!out) return fail(VS_EINVAL, "null out");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(VS_EINVAL, "device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    vs_ctx *c = new vs_ctx();
+    c->device = device;
+    const char *ng = std::getenv("VS_NO_GRAPH");
+    c->use_graph = !(ng && ng[0] == '1');
+    *out = c;
+    return VS_OK;
+}
+
+vs_status vs_ctx_destroy(vs_ctx *ctx) {
+    if (!ctx) return VS_OK;
+    hipSetDevice(ctx->device);
+    for (auto &kv : ctx->batches) {
+        free_batch_graph(*kv.second);
+        if (kv.second->owned) hipFree(kv.second->owned);
+    }
+    ctx->batches.clear();
+    std::vector<vs_session *> ss = ctx->sessions;
+    for (vs_session *s : ss) vs_session_destroy(s);
+    delete ctx;
+    return VS_OK;
+}
+
+size_t vs_session_bytes(const vs_session_desc *d) {
+    if (check_desc(d) != VS_OK) return 0;
+    return make_layout(*d).total;
+}
+
+vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const double T_gc0[16], vs_session **out) {
+    if (!ctx || !out) return fail(VS_EINVAL, "null argument");
+    vs_status cs = check_desc(desc);
+    if (cs != VS_OK) return cs;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<vs_session> s(new vs_session());
+    s->ctx = ctx;
+    s->d = *desc;
+    s->L = make_layout(*desc);
+    const Layout &L = s->L;
+    hipError_t e = hipMalloc((void **)&s->mem, L.total);
+    if (e != hipSuccess) return fail(VS_ENOMEM, "hipMalloc(%zu) failed: %s", L.total, hipGetErrorString(e));
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->last = s->stream;
+    char *m = s->mem;
+    SessParams &P = s->hp;
+    std::memset(&P, 0, sizeof(P));
+    const vs_session_desc &d = *desc;
+    // per-level geometry; intrinsics halve per level with c <- (c - 0.5)/2
+    // (pose_tracker.cu:764-776, tsdf_renderer.cu:1611-1623)
+    float fx = d.fx, fy = d.fy, px = d.cx, py = d.cy;
+    for (int l = 0; l < kMaxLevels; ++l) {
+        LevelDesc &ld = P.lev[l];
+        ld.W = d.width >> l;
+        ld.H = d.height >> l;
+        ld.tiles_x = (ld.W + kTileW - 1) / kTileW;
+        ld.tiles_y = (ld.H + kTileH - 1) / kTileH;
+        ld.K[0] = fx;
+        ld.K[1] = fy;
+        ld.K[2] = px;
+        ld.K[3] = py;
+        fx = 0.5f * fx;
+        fy = 0.5f * fy;
+        px = 0.5f * (px - 0.5f);
+        py = 0.5f * (py - 0.5f);
+        P.iters[l] = l < d.levels ? d.iters[l] : 0;
+    }
+    P.levels = d.levels;
+    P.icp_dist2 = d.icp_dist2;
+    P.icp_min_valid = d.icp_min_valid;
+    P.max_motion_m = d.max_motion_m;
+    P.max_motion_deg = d.max_motion_deg;
+    for (int k = 0; k < 3; ++k) {
+        P.vdim[k] = d.vol_dim[k];
+        P.bdim[k] = d.vol_dim[k] / kBrick;
+        P.origin[k] = d.origin[k];
+    }
+    P.vox_inv = 1.0f / d.voxel;
+    P.mu = d.mu;
+    P.max_w = d.max_weight;
+    // cube centre = origin + dim*voxel/2 (z_yooji_object_scanner.cpp:306); radius = |origin - centre|
+    for (int k = 0; k < 3; ++k) P.cube_cen[k] = d.origin[k] + (float)d.vol_dim[k] * d.voxel / 2.0f;
+    {
+        const float ex = d.origin[0] - P.cube_cen[0], ey = d.origin[1] - P.cube_cen[1], ez = d.origin[2] - P.cube_cen[2];
+        P.r_cube = sqrtf(ex * ex + ey * ey + ez * ez);
+    }
+    P.step_scale = 0.9f * P.mu * P.vox_inv;
+    P.total_max = 2.0f * P.r_cube * P.vox_inv;
+    P.max_iter = (int)(4.0f * P.total_max / std::min(P.step_scale, 1.0f)) + 64;
+    // bilateral spatial weights exactly as updateGaussian (img_proc.cu:275-286)
+    for (int i = 0; i < 2 * d.bil_radius + 1; ++i) {
+        float x = (float)(i - d.bil_radius);
+        P.bil_g[i] = expf(-(x * x) / (2 * d.bil_sigma_s * d.bil_sigma_s));
+    }
+    P.bil_inv2s2 = 1.0f / (2.0f * d.bil_sigma_r * d.bil_sigma_r);
+    P.bil_radius = d.bil_radius;
+    P.raw = (const uint16_t *)(m + L.raw);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        P.filt[l] = (float *)(m + L.filt[l]);
+        P.mvtx[l] = (float4 *)(m + L.mvtx[l]);
+        P.mnrm[l] = (float4 *)(m + L.mnrm[l]);
+        P.mimg[l] = (uint8_t *)(m + L.mimg[l]);
+    }
+    P.tsdf = m + L.tsdf;
+    P.weight = m + L.weight;
+    P.bvis = (uint32_t *)(m + L.bvis);
+    P.bocc = (uint32_t *)(m + L.bocc);
+    P.icp_part = (double *)(m + L.icp_part);
+    P.st = (SessState *)(m + L.state);
+    P.tsdf_type = d.tsdf_type;
+    P.weight_type = d.weight_type;
+    P.sid = (int)ctx->sessions.size();
+    s->st = P.st;
+    // self batch
+    Batch &B = s->self;
+    B.n = 1;
+    B.tt = d.tsdf_type;
+    B.wt = d.weight_type;
+    B.levels = d.levels;
+    for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = P.iters[l];
+    B.dP = (SessParams *)(m + L.params);
+    std::vector<WorkItem> wk;
+    for (int l = 0; l < kMaxLevels; ++l) {
+        B.dWork[l] = (WorkItem *)(m + L.work[l]);
+        B.nWork[l] = L.tiles[l];
+        wk.resize(std::max(L.tiles[l], 1));
+        for (int t = 0; t < L.tiles[l]; ++t) wk[t] = {0, t};
+        HIP_TRY(hipMemcpy(B.dWork[l], wk.data(), wk.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    }
+    B.bl.items = (int2 *)(m + L.blist);
+    B.bl.count = (unsigned *)(m + L.bcount);
+    B.bl.capacity = L.nbricks;
+    B.integ_grid = (int)std::min<uint32_t>(L.nbricks, 2048);
+    B.members = {s.get()};
+    HIP_TRY(hipMemcpy(B.dP, &P, sizeof(P), hipMemcpyHostToDevice));
+    // state: pose, counters; volume F = 1, W = 0
+    HIP_TRY(hipMemset(m + L.state, 0, sizeof(SessState)));
+    HIP_TRY(hipMemset(m + L.bcount, 0, 16));
+    for (int l = 0; l < kMaxLevels; ++l) {
+        const size_t n = (size_t)(d.width >> l) * (d.height >> l);
+        HIP_TRY(hipMemset(m + L.filt[l], 0, n * 4));
+        HIP_TRY(hipMemset(m + L.mimg[l], 0, n));
+        std::vector<float4> inval(n, make_float4(-100.f, -100.f, -100.f, 0.f));
+        HIP_TRY(hipMemcpy(m + L.mvtx[l], inval.data(), n * 16, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(m + L.mnrm[l], inval.data(), n * 16, hipMemcpyHostToDevice));
+    }
+    vs_session *raw = s.release();
+    ctx->sessions.push_back(raw);
+    vs_status r = vs_session_reset(raw, T_gc0, nullptr);
+    if (r != VS_OK) {
+        vs_session_destroy(raw);
+        return r;
+    }
+    HIP_TRY(hipStreamSynchronize(raw->stream));
+    *out = raw;
+    return VS_OK;
+}
+
+vs_status vs_session_destroy(vs_session *s) {
+    if (!s) return VS_OK;
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->stream);
+    drop_batches_with(s->ctx, s);
+    free_batch_graph(s->self);
+    for (auto &e : s->ev)
+        if (e) hipEventDestroy(e);
+    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->mem) hipFree(s->mem);
+    auto &v = s->ctx->sessions;
+    v.erase(std::remove(v.begin(), v.end(), s), v.end());
+    delete s;
+    return VS_OK;
+}
+
+vs_status vs_session_reset(vs_session *s, const double T_gc0[16], vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    launch_volume_init(s->self.dP, s->L.nvox, s->d.tsdf_type, s->d.weight_type, s->L.nwords, h);
+    SessState init{};
+    for (int k = 0; k < 16; ++k) init.T_gc[k] = T_gc0 ? T_gc0[k] : ((k % 5) == 0 ? 1.0 : 0.0);
+    init.conv_level = -1;
+    HIP_TRY(hipMemcpyAsync(s->st, &init, sizeof(init), hipMemcpyHostToDevice, h));
+    launch_pose_derived(s->self.dP, h);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h));
+    return VS_OK;
+}
+
+vs_status vs_session_desc_get(const vs_session *s, vs_session_desc *out) {
+    if (!s || !out) return fail(VS_EINVAL, "null argument");
+    *out = s->d;
+    return VS_OK;
+}
+
+vs_status vs_session_sync(vs_session *s) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    return sync_last(s);
+}
+
+vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    return copy_depth(s, depth_mm, is_host, pick(s, stream));
+}
+
+vs_status vs_bilateral(vs_session *s, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    launch_bilateral(s->self.dP, s->self.dWork[0], s->self.nWork[0], h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_pyramid(vs_session *s, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    for (int l = 1; l < s->d.levels; ++l) launch_pyr_down(s->self.dP, s->self.dWork[l], s->self.nWork[l], l, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_vertex_normal_maps(vs_session *s, int level, float *vtx_dev, float *nrm_dev, vs_stream stream) {
+    if (!s || !vtx_dev || !nrm_dev) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    hipStream_t h = pick(s, stream);
+    launch_vnmap(s->self.dP, s->self.dWork[level], s->self.nWork[level], level, vtx_dev, nrm_dev, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_track(vs_session *s, const double *T_01_init, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    const Batch &B = s->self;
+    launch_track_setup_forced(B.dP, 1, T_01_init, h);
+    for (int l = B.levels - 1; l >= 0; --l)
+        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], l, it, h);
+    launch_track_finalize(B.dP, 1, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_integrate(vs_session *s, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    const Batch &B = s->self;
+    launch_fuse_ok(B.dP, 1, h);
+    launch_vis_clear(B.dP, 1, B.bl, h);
+    launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, h);
+    launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_raycast(vs_session *s, int level, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    hipStream_t h = pick(s, stream);
+    const Batch &B = s->self;
+    launch_fuse_ok(B.dP, 1, h);
+    launch_raycast(B.dP, B.dWork[level], B.nWork[level], level, B.tt, B.wt, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    vs_status r = copy_depth(s, depth_mm, is_host, h);
+    if (r != VS_OK) return r;
+    if (s->timing) {
+        for (auto &e : s->ev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
+    }
+    return run_step(s->ctx, s->self, h, s->timing ? s : nullptr);
+}
+
+vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t *const *depths, int n, int is_host,
+                        vs_stream stream) {
+    if (!ctx || !sessions || !depths || n <= 0) return fail(VS_EINVAL, "bad batch arguments");
+    std::vector<vs_session *> key(sessions, sessions + n);
+    for (int i = 0; i < n; ++i) {
+        if (!key[i] || key[i]->ctx != ctx) return fail(VS_EINVAL, "session %d not in context", i);
+        if (key[i]->d.tsdf_type != key[0]->d.tsdf_type || key[i]->d.weight_type != key[0]->d.weight_type)
+            return fail(VS_EINVAL, "sessions of one batch must share storage types");
+        for (int j = 0; j < i; ++j)
+            if (key[j] == key[i]) return fail(VS_EINVAL, "session repeated in batch");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t h = stream ? (hipStream_t)stream : key[0]->stream;
+    auto it = ctx->batches.find(key);
+    Batch *B;
+    if (it == ctx->batches.end()) {
+        std::unique_ptr<Batch> nb(new Batch());
+        nb->n = n;
+        nb->tt = key[0]->d.tsdf_type;
+        nb->wt = key[0]->d.weight_type;
+        nb->members = key;
+        std::vector<SessParams> hp(n);
+        std::vector<WorkItem> wk[kMaxLevels];
+        uint32_t cap = 0;
+        for (int i = 0; i < n; ++i) {
+            hp[i] = key[i]->hp;
+            nb->levels = std::max(nb->levels, key[i]->d.levels);
+            for (int l = 0; l < kMaxLevels; ++l) {
+                nb->max_iters[l] = std::max(nb->max_iters[l], key[i]->hp.iters[l]);
+                for (int t = 0; t < key[i]->L.tiles[l]; ++t) wk[l].push_back({i, t});
+            }
+            cap += key[i]->L.nbricks;
+        }
+        size_t off = 0, offP, offW[kMaxLevels], offB, offC;
+        offP = off; off = align_up(off + n * sizeof(SessParams));
+        for (int l = 0; l < kMaxLevels; ++l) {
+            offW[l] = off;
+            off = align_up(off + std::max<size_t>(wk[l].size(), 1) * sizeof(WorkItem));
+        }
+        offB = off; off = align_up(off + (size_t)cap * sizeof(int2));
+        offC = off; off = align_up(off + 16);
+        char *mem = nullptr;
+        hipError_t e = hipMalloc((void **)&mem, off);
+        if (e != hipSuccess) return fail(VS_ENOMEM, "batch allocation (%zu bytes): %s", off, hipGetErrorString(e));
+        nb->owned = mem;
+        nb->dP = (SessParams *)(mem + offP);
+        HIP_TRY(hipMemcpy(nb->dP, hp.data(), n * sizeof(SessParams), hipMemcpyHostToDevice));
+        for (int l = 0; l < kMaxLevels; ++l) {
+            nb->dWork[l] = (WorkItem *)(mem + offW[l]);
+            nb->nWork[l] = (int)wk[l].size();
+            if (!wk[l].empty())
+                HIP_TRY(hipMemcpy(nb->dWork[l], wk[l].data(), wk[l].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        }
+        nb->bl.items = (int2 *)(mem + offB);
+        nb->bl.count = (unsigned *)(mem + offC);
+        nb->bl.capacity = cap;
+        HIP_TRY(hipMemset(nb->bl.count, 0, 16));
+        nb->integ_grid = (int)std::min<uint32_t>(cap, 8192);
+        B = nb.get();
+        ctx->batches[key] = std::move(nb);
+    } else {
+        B = it->second.get();
+    }
+    for (int i = 0; i < n; ++i) {
+        vs_status r = copy_depth(key[i], depths[i], is_host, h);
+        if (r != VS_OK) return r;
+        key[i]->last = h;
+    }
+    return run_step(ctx, *B, h, nullptr);
+}
+
+vs_status vs_get_frame_status(vs_session *s, int *last_status, int *frames_fused, int *icp_iters) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    SessState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof(st), hipMemcpyDeviceToHost));
+    if (last_status) *last_status = st.last_status;
+    if (frames_fused) *frames_fused = st.frames;
+    if (icp_iters) *icp_iters = st.solves;
+    return VS_OK;
+}
+
+vs_status vs_get_pose(vs_session *s, double T_gc[16]) {
+    if (!s || !T_gc) return fail(VS_EINVAL, "null argument");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    HIP_TRY(hipMemcpy(T_gc, s->st->T_gc, 16 * sizeof(double), hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+vs_status vs_set_pose(vs_session *s, const double T_gc[16], vs_stream stream) {
+    if (!s || !T_gc) return fail(VS_EINVAL, "null argument");
+    hipStream_t h = pick(s, stream);
+    HIP_TRY(hipStreamSynchronize(h));
+    HIP_TRY(hipMemcpy(s->st->T_gc, T_gc, 16 * sizeof(double), hipMemcpyHostToDevice));
+    launch_pose_derived(s->self.dP, h);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h));
+    return VS_OK;
+}
+
+vs_status vs_get_filtered(vs_session *s, int level, float *host) {
+    if (!s || !host) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
+    HIP_TRY(hipMemcpy(host, s->hp.filt[level], n * 4, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+vs_status vs_set_filtered(vs_session *s, int level, const float *host) {
+    if (!s || !host) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
+    HIP_TRY(hipMemcpy(s->hp.filt[level], host, n * 4, hipMemcpyHostToDevice));
+    return VS_OK;
+}
+
+vs_status vs_get_maps(vs_session *s, int level, float *depth, float *vtx, float *nrm, uint8_t *img) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
+    std::vector<float4> v(n), m(n);
+    HIP_TRY(hipMemcpy(v.data(), s->hp.mvtx[level], n * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(m.data(), s->hp.mnrm[level], n * 16, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        if (depth) depth[i] = (v[i].x == -100.0f) ? 0.0f : v[i].z;
+        if (vtx) {
+            vtx[3 * i] = v[i].x;
+            vtx[3 * i + 1] = v[i].y;
+            vtx[3 * i + 2] = v[i].z;
+        }
+        if (nrm) {
+            nrm[3 * i] = m[i].x;
+            nrm[3 * i + 1] = m[i].y;
+            nrm[3 * i + 2] = m[i].z;
+        }
+    }
+    if (img) HIP_TRY(hipMemcpy(img, s->hp.mimg[level], n, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+vs_status vs_set_maps(vs_session *s, int level, const float *vtx, const float *nrm) {
+    if (!s || !vtx || !nrm) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
+    std::vector<float4> v(n), m(n);
+    for (size_t i = 0; i < n; ++i) {
+        v[i] = make_float4(vtx[3 * i], vtx[3 * i + 1], vtx[3 * i + 2], 0.0f);
+        m[i] = make_float4(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.0f);
+    }
+    HIP_TRY(hipMemcpy(s->hp.mvtx[level], v.data(), n * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->hp.mnrm[level], m.data(), n * 16, hipMemcpyHostToDevice));
+    return VS_OK;
+}
+
+vs_status vs_volume_download(vs_session *s, float *tsdf, float *weight, uint8_t *brick_mask) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t N = s->L.nvox;
+    if (tsdf || weight) {
+        float *tmp = nullptr;
+        HIP_TRY(hipMalloc((void **)&tmp, 2 * N * sizeof(float)));
+        launch_volume_to_linear(s->self.dP, N, s->d.tsdf_type, s->d.weight_type, tmp, tmp + N, s->stream);
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess && tsdf) e = hipMemcpy(tsdf, tmp, N * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && weight) e = hipMemcpy(weight, tmp + N, N * 4, hipMemcpyDeviceToHost);
+        hipFree(tmp);
+        if (e != hipSuccess) return fail(VS_EHIP, "volume download: %s", hipGetErrorString(e));
+    }
+    if (brick_mask) {
+        std::vector<uint32_t> bits(s->L.nwords);
+        HIP_TRY(hipMemcpy(bits.data(), s->hp.bvis, bits.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t b = 0; b < s->L.nbricks; ++b) brick_mask[b] = (bits[b >> 5] >> (b & 31)) & 1u;
+    }
+    return VS_OK;
+}
+
+vs_status vs_volume_upload(vs_session *s, const float *tsdf, const float *weight) {
+    if (!s || !tsdf || !weight) return fail(VS_EINVAL, "null argument");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t N = s->L.nvox;
+    float *tmp = nullptr;
+    HIP_TRY(hipMalloc((void **)&tmp, 2 * N * sizeof(float)));
+    hipError_t e = hipMemcpy(tmp, tsdf, N * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(tmp + N, weight, N * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_volume_from_linear(s->self.dP, N, s->d.tsdf_type, s->d.weight_type, tmp, tmp + N, s->stream);
+        launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.weight_type, s->stream);
+        e = hipStreamSynchronize(s->stream);
+    }
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(VS_EHIP, "volume upload: %s", hipGetErrorString(e));
+    return VS_OK;
+}
+
+vs_status vs_icp_system(vs_session *s, int level, const float T_21[16], double ATA[36], double ATb[6],
+                        long long *valid, uint8_t *mask_host) {
+    if (!s || !T_21) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const int nt = s->L.tiles[level];
+    double *part = (double *)(s->mem + s->L.diag_part);
+    uint8_t *mask = mask_host ? (uint8_t *)(s->mem + s->L.diag_mask) : nullptr;
+    launch_icp_system(s->self.dP, s->self.dWork[level], nt, level, T_21, part, mask, s->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<double> h((size_t)nt * kIcpVals);
+    HIP_TRY(hipMemcpy(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost));
+    double tot[kIcpVals] = {0};
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < kIcpVals; ++k) tot[k] += h[(size_t)t * kIcpVals + k];
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            if (ATA) ATA[i * 6 + j] = ATA[j * 6 + i] = tot[k];
+            ++k;
+        }
+    for (int i = 0; i < 6; ++i)
+        if (ATb) ATb[i] = tot[k + i];
+    if (valid) *valid = (long long)llround(tot[27]);
+    if (mask_host) {
+        const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
+        HIP_TRY(hipMemcpy(mask_host, mask, n, hipMemcpyDeviceToHost));
+    }
+    return VS_OK;
+}
+
+vs_status vs_session_set_timing(vs_session *s, int enable) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    s->timing = enable != 0;
+    return VS_OK;
+}
+
+vs_status vs_get_stage_ms(vs_session *s, float ms[5]) {
+    if (!s || !ms) return fail(VS_EINVAL, "null argument");
+    if (!s->timing || !s->ev[0]) return fail(VS_EINVAL, "timing not enabled");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    for (int i = 0; i < 5; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], s->ev[i], s->ev[i + 1]));
+    return VS_OK;
+}
+
+vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long long *updated_voxels) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    SessState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof(st), hipMemcpyDeviceToHost));
+    if (visible_bricks) *visible_bricks = st.vis_count;
+    if (updated_voxels) *updated_voxels = (long long)st.upd_count;
+    return VS_OK;
+}
+
+/* Tap counts of one instrumented raycast at `level` with the current pose:
+ * [nearest taps, trilinear evaluations, hits]. Used to compute the algorithmic
+ * bytes of the raycast; not part of the reference surface. */
+vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]) {
+    if (!s || !counts) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    unsigned long long *d = (unsigned long long *)(s->mem + s->L.stats);
+    HIP_TRY(hipMemset(d, 0, 32));
+    const Batch &B = s->self;
+    launch_fuse_ok(B.dP, 1, s->stream);
+    launch_raycast_stats(B.dP, B.dWork[level], B.nWork[level], level, B.tt, B.wt, d, s->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(counts, d, 24, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// vs_device.hpp — session layout in HBM and the FP32 device helpers shared by
+// every kernel of the fusion pipeline.
+//
+// All HIP sources are compiled with -ffp-contract=off: the helpers below
+// evaluate index- and mask-deciding expressions in FP32 in the reference's
+// operation order (STAC/Util/_yooji_cuda_math.h:205-302), one IEEE rounding
+// per operation, so indices and masks are bit-identical to any other
+// implementation that does the same. Value arithmetic that decides nothing
+// uses fmaf where it pays.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vs {
+
+constexpr int kMaxLevels = 3;
+constexpr int kBrick = 8;           // brick edge (voxels); a brick is 512 voxels, stored contiguously
+constexpr int kBrickVox = 512;
+constexpr int kTileW = 32;          // image tile of one 256-thread workgroup: 32 x 8 pixels
+constexpr int kTileH = 8;
+constexpr int kTileThreads = kTileW * kTileH;
+constexpr int kIcpVals = 29;        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
+
+struct LevelDesc {
+    int W, H;
+    int tiles_x, tiles_y;
+    float K[4];  // fx, fy, cx, cy
+};
+
+// Mutable per-session state; lives in HBM, touched only by kernels.
+struct alignas(16) SessState {
+    double T_gc[16];   // FP64 master pose (world -> camera)
+    double T_cg[16];
+    double T_12[16];   // tracker increment camera t0 -> t1
+    double T_21[16];
+    float T_gc_f[16];  // FP32 copies the kernels consume
+    float T_cg_f[16];
+    float T_21_f[16];
+    float cam[3];      // camera centre (FP32, from T_gc_f)
+    float ncc[3];      // unit vector camera -> cube centre
+    float dist_cc;     // |cube centre - camera|
+    float cos_max;     // raycast cone threshold
+    int frames;        // frames fused so far
+    int do_track;      // this frame runs the tracker
+    int icp_failed;    // 0 running, else a vs_status code
+    int conv_level;    // level whose iterations converged (-1 none)
+    int fuse_ok;       // this frame integrates and raycasts
+    int last_status;   // status of the last frame
+    int solves;        // Gauss-Newton solves of the last frame
+    int force_single;  // T_init given: only level 0
+    unsigned icp_counter;       // last-block counter of the ICP reduction
+    unsigned vis_count;         // visible bricks of the last integrate
+    unsigned long long upd_count;  // voxels updated by the last integrate
+    double last_ATA[36];        // normal equations of the last solve (diagnostic)
+    double last_ATb[6];
+    double last_valid;
+};
+
+// Immutable per-session parameters plus buffer pointers; one per session in
+// a device array, so a launch can serve a batch of sessions.
+struct alignas(16) SessParams {
+    LevelDesc lev[kMaxLevels];
+    int levels;
+    int iters[kMaxLevels];
+    float icp_dist2;
+    int icp_min_valid;
+    float max_motion_m, max_motion_deg;
+    int vdim[3];        // voxels per axis
+    int bdim[3];        // bricks per axis
+    float origin[3];
+    float vox_inv;      // 1.0f / voxel (FP32)
+    float mu, max_w;
+    float cube_cen[3];
+    float r_cube, step_scale, total_max;
+    int max_iter;
+    float bil_g[8];     // 1-D spatial weights
+    float bil_inv2s2;   // 1 / (2 sigma_r^2)
+    int bil_radius;
+    // buffers (device)
+    const uint16_t *raw;     // W*H depth (mm), level-0 size
+    float *filt[kMaxLevels];
+    float4 *mvtx[kMaxLevels];  // model vertex (xyz), -100 invalid
+    float4 *mnrm[kMaxLevels];  // model normal (xyz), -100 invalid
+    uint8_t *mimg[kMaxLevels];
+    void *tsdf;              // brick-major values
+    void *weight;            // brick-major weights
+    uint32_t *bvis;          // visible-brick bits of the current frame
+    uint32_t *bocc;          // bricks holding any weight > 0 (monotone)
+    double *icp_part;        // ICP block partials (kIcpVals per tile of level 0)
+    SessState *st;
+    int tsdf_type, weight_type;
+    int sid;                 // index in the context's table
+};
+
+struct WorkItem {
+    int sess;  // index into the SessParams array of the launch
+    int tile;  // tile (image stages) within the session's level
+};
+
+// ---------------------------------------------------------------------------
+// FP32 helpers, reference operation order
+// ---------------------------------------------------------------------------
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 scale(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 normalised(f3 v) {
+    float n = 1.0f / sqrtf(dot(v, v));
+    return {v.x * n, v.y * n, v.z * n};
+}
+__device__ __forceinline__ f3 transform(const float *T, f3 p) {
+    return {T[0] * p.x + T[1] * p.y + T[2] * p.z + T[3], T[4] * p.x + T[5] * p.y + T[6] * p.z + T[7],
+            T[8] * p.x + T[9] * p.y + T[10] * p.z + T[11]};
+}
+__device__ __forceinline__ f3 back_project(float u, float v, const float *K, float d) {
+    return {d * (u - K[2]) / K[0], d * (v - K[3]) / K[1], d};
+}
+__device__ __forceinline__ void project(f3 p, const float *K, float &u, float &v) {
+    u = K[0] * p.x / p.z + K[2];
+    v = K[1] * p.y / p.z + K[3];
+}
+// ROUNDF (reference _yooji_cuda_defines.h:53) with out-of-range -> false.
+__device__ __forceinline__ bool roundf_ref(float v, int &out) {
+    if (!(v > -2147483520.0f && v < 2147483520.0f)) return false;
+    out = (v < 0.0f) ? (int)(v - 0.5f) : (int)(v + 0.5f);
+    return true;
+}
+__device__ __forceinline__ float depth_m(uint16_t mm) { return (float)mm * 0.001f; }
+
+__device__ __forceinline__ bool world_to_voxel(const SessParams &P, f3 p, f3 &vox) {
+    vox.x = (p.x - P.origin[0]) * P.vox_inv - 0.5f;
+    vox.y = (p.y - P.origin[1]) * P.vox_inv - 0.5f;
+    vox.z = (p.z - P.origin[2]) * P.vox_inv - 0.5f;
+    return !(vox.x < -0.5f || vox.x >= (float)P.vdim[0] - 0.5f || vox.y < -0.5f || vox.y >= (float)P.vdim[1] - 0.5f ||
+             vox.z < -0.5f || vox.z >= (float)P.vdim[2] - 0.5f);
+}
+__device__ __forceinline__ f3 voxel_to_world(const SessParams &P, f3 vox) {
+    return {(vox.x + 0.5f) / P.vox_inv + P.origin[0], (vox.y + 0.5f) / P.vox_inv + P.origin[1],
+            (vox.z + 0.5f) / P.vox_inv + P.origin[2]};
+}
+
+// Brick-major voxel address: brick (x>>3, y>>3, z>>3), then z,y,x inside.
+__device__ __forceinline__ size_t vox_index(const SessParams &P, int x, int y, int z) {
+    size_t b = ((size_t)(z >> 3) * P.bdim[1] + (y >> 3)) * P.bdim[0] + (x >> 3);
+    return b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
+}
+__device__ __forceinline__ int brick_of(const SessParams &P, int x, int y, int z) {
+    return ((z >> 3) * P.bdim[1] + (y >> 3)) * P.bdim[0] + (x >> 3);
+}
+
+// Storage conversions for the two volume forms.
+__device__ __forceinline__ float ld_val(const float *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ld_val(const __half *p, size_t i) { return __half2float(p[i]); }
+__device__ __forceinline__ void st_val(float *p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void st_val(__half *p, size_t i, float v) { p[i] = __float2half_rn(v); }
+__device__ __forceinline__ float ld_w(const float *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ld_w(const uint8_t *p, size_t i) { return (float)p[i]; }
+__device__ __forceinline__ void st_w(float *p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void st_w(uint8_t *p, size_t i, float v) { p[i] = (uint8_t)v; }
+
+// Wave64 sum via cross-lane shuffles (ds_swizzle/DPP under the hood).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace vs

@@ -1,0 +1,446 @@
+// vs_icp.hip — projective point-to-plane ICP with the Gauss-Newton loop on
+// the device (A8-A12).
+//
+// Reference: LGKvPoseTracker::tp_Track_Pose (z_yooji_cuda_pose_tracker.cu:790-1042)
+// launches one summand+reduction kernel per iteration, copies ~200 KB of
+// block partials to the host, sums them in FP32, solves on the host and copies
+// the pose back. Here each iteration is one launch whose last workgroup (per
+// session) sums the partials in FP64 in tile order, solves the 6x6 system
+// (Cholesky, FP64), applies the SE(3) update and publishes the FP32 pose the
+// next launch reads. Convergence (|x|_inf < 5e-7) and failure are flags the
+// following launches test, so the host enqueues a fixed launch sequence that
+// can be captured in a graph.
+#include "vs_device.hpp"
+#include "vs_launch.hpp"
+
+namespace vs {
+
+struct Mat16d {
+    double m[16];
+};
+struct Mat16f {
+    float m[16];
+};
+
+// ---------------------------------------------------------------------------
+// FP64 pose algebra (one thread)
+// ---------------------------------------------------------------------------
+__device__ void matmul4d(const double *A, const double *B, double *C) {
+    double t[16];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < 4; ++k) s = fma(A[r * 4 + k], B[k * 4 + c], s);
+            t[r * 4 + c] = s;
+        }
+    for (int k = 0; k < 16; ++k) C[k] = t[k];
+}
+// Inverse of a rigid transform: [R^T, -R^T t]. Equals the reference's general
+// 4x4 inverse (d_im_Inverse_Matrix_4x4, _yooji_math_inlines.h:404-465) on
+// rigid inputs.
+__device__ void rigid_inverse(const double *T, double *Ti) {
+    double o[16];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) o[r * 4 + c] = T[c * 4 + r];
+        o[r * 4 + 3] = -(T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7] + T[2 * 4 + r] * T[11]);
+    }
+    o[12] = o[13] = o[14] = 0.0;
+    o[15] = 1.0;
+    for (int k = 0; k < 16; ++k) Ti[k] = o[k];
+}
+// SE(3) exponential of x = [w; v] (the twist update of
+// z_uitme_Update_Incremental_Tracking_with_Matrix_Exponential, pose_tracker.cu:50-149),
+// with a series branch for small |w| (the reference divides by |w|).
+__device__ void expmap(const double *x, double *T) {
+    const double w0 = x[0], w1 = x[1], w2 = x[2];
+    const double th2 = w0 * w0 + w1 * w1 + w2 * w2, th = sqrt(th2);
+    double A, B, C;
+    if (th < 1e-4) {
+        A = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+        B = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+        C = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
+    } else {
+        A = sin(th) / th;
+        B = (1.0 - cos(th)) / th2;
+        C = (th - sin(th)) / (th2 * th);
+    }
+    const double W[9] = {0, -w2, w1, w2, 0, -w0, -w1, w0, 0};
+    double W2[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) W2[r * 3 + c] = W[r * 3] * W[c] + W[r * 3 + 1] * W[3 + c] + W[r * 3 + 2] * W[6 + c];
+    for (int r = 0; r < 3; ++r) {
+        double t = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double I = (r == c) ? 1.0 : 0.0;
+            T[r * 4 + c] = I + A * W[r * 3 + c] + B * W2[r * 3 + c];
+            t += (I + B * W[r * 3 + c] + C * W2[r * 3 + c]) * x[3 + c];
+        }
+        T[r * 4 + 3] = t;
+    }
+    T[12] = T[13] = T[14] = 0.0;
+    T[15] = 1.0;
+}
+// Cholesky and triangular solves with the reference's pivot rule
+// (d_lld_LL_Decomposition / d_sls_Solve_Linear_System_using_LLD,
+// _yooji_cuda_math.h:489-592): entries below a pivot < 1e-8 are zeroed and the
+// solve fails if any pivot is below 1e-8 (or not a number).
+__device__ bool cholesky_solve(const double *A, const double *b, double *x) {
+    double L[36];
+    for (int k = 0; k < 36; ++k) L[k] = 0.0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j <= i; ++j) {
+            if (i == j) {
+                double s = 0.0;
+                for (int k = 0; k < j; ++k) s += L[j * 6 + k] * L[j * 6 + k];
+                L[j * 6 + j] = sqrt(A[j * 6 + j] - s);
+            } else if (fabs(L[j * 6 + j]) >= 1e-8) {
+                double s = 0.0;
+                for (int k = 0; k < j; ++k) s += L[i * 6 + k] * L[j * 6 + k];
+                L[i * 6 + j] = (A[i * 6 + j] - s) / L[j * 6 + j];
+            } else {
+                L[i * 6 + j] = 0.0;
+            }
+        }
+    for (int i = 0; i < 6; ++i)
+        if (!(fabs(L[i * 6 + i]) >= 1e-8)) return false;
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < i; ++k) s += L[i * 6 + k] * y[k];
+        y[i] = (b[i] - s) / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = 0.0;
+        for (int k = i + 1; k < 6; ++k) s += L[k * 6 + i] * x[k];
+        x[i] = (y[i] - s) / L[i * 6 + i];
+    }
+    return true;
+}
+
+__device__ void publish_T21(SessState *st) {
+    rigid_inverse(st->T_12, st->T_21);
+    for (int k = 0; k < 16; ++k) st->T_21_f[k] = (float)st->T_21[k];
+}
+
+// ---------------------------------------------------------------------------
+// per-pixel summand (d_cssfp_Compute_Single_Summand_with_Forward_Params,
+// pose_tracker.cu:279-395; bilinear taps d_giv/d_gin, _yooji_cuda_math.h:365-481)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool bilinear_taps(const float4 *m, int W, float u, float v, int &x0, int &y0, float &rx,
+                                              float &ry, f3 &out) {
+    x0 = (int)u;
+    y0 = (int)v;
+    rx = u - (float)x0;
+    ry = v - (float)y0;
+    f3 t1{0, 0, 0}, t2{0, 0, 0}, t3{0, 0, 0}, t4{0, 0, 0};
+    const size_t i0 = (size_t)y0 * W + x0;
+    float4 a = m[i0];
+    if (a.x == -100.0f) return false;
+    t1 = {a.x, a.y, a.z};
+    if (rx > 0.0f) {
+        a = m[i0 + 1];
+        if (a.x == -100.0f) return false;
+        t2 = {a.x, a.y, a.z};
+    }
+    if (ry > 0.0f) {
+        a = m[i0 + W];
+        if (a.x == -100.0f) return false;
+        t3 = {a.x, a.y, a.z};
+    }
+    if (rx > 0.0f && ry > 0.0f) {
+        a = m[i0 + W + 1];
+        if (a.x == -100.0f) return false;
+        t4 = {a.x, a.y, a.z};
+    }
+    const float ix = 1.0f - rx, iy = 1.0f - ry;
+    out = add(scale(iy, add(scale(ix, t1), scale(rx, t2))), scale(ry, add(scale(ix, t3), scale(rx, t4))));
+    return true;
+}
+
+__device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, int y, const float *T21, float A[6],
+                                            float &b) {
+    const LevelDesc &L = P.lev[l];
+    const int W = L.W, H = L.H;
+    const float td1 = P.filt[l][(size_t)y * W + x];
+    if (td1 < 1.0e-6f) return false;
+    const f3 p = transform(T21, back_project((float)x, (float)y, L.K, td1));
+    float u, v;
+    project(p, L.K, u, v);
+    if (!(u >= 0.0f && u <= float(W - 1) && v >= 0.0f && v <= float(H - 1))) return false;
+    int x0, y0;
+    float rx, ry;
+    f3 v0, n;
+    if (!bilinear_taps(P.mvtx[l], W, u, v, x0, y0, rx, ry, v0)) return false;
+    const float dx = v0.x - p.x, dy = v0.y - p.y, dz = v0.z - p.z;
+    if (dx * dx + dy * dy + dz * dz > P.icp_dist2) return false;
+    if (!bilinear_taps(P.mnrm[l], W, u, v, x0, y0, rx, ry, n)) return false;
+    n = normalised(n);
+    A[0] = p.z * n.y + (-p.y) * n.z;
+    A[1] = (-p.z) * n.x + p.x * n.z;
+    A[2] = p.y * n.x + (-p.x) * n.y;
+    A[3] = -n.x;
+    A[4] = -n.y;
+    A[5] = -n.z;
+    b = n.x * dx + n.y * dy + n.z * dz;
+    return true;
+}
+
+// 29 sums of one pixel: JtJ upper triangle (21), Jtr (6), count, r^2.
+__device__ __forceinline__ void pixel_terms(bool ok, const float A[6], float b, float v[kIcpVals]) {
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) v[k++] = ok ? A[i] * A[j] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[k++] = ok ? A[i] * b : 0.0f;
+    v[k++] = ok ? 1.0f : 0.0f;
+    v[k++] = ok ? b * b : 0.0f;
+}
+
+// Block sum of the 29 terms: FP32 within each wave (64 pixels), FP64 across
+// the 4 waves. Result in red64 (shared), valid after the call for all threads.
+__device__ __forceinline__ void block_terms(float v[kIcpVals], double *red64) {
+    __shared__ float wsum[kTileThreads / 64][kIcpVals];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kIcpVals; ++k) {
+        float s = wave_sum(v[k]);
+        if (lane == 0) wsum[wv][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kIcpVals) {
+        double s = 0.0;
+        for (int w = 0; w < kTileThreads / 64; ++w) s += (double)wsum[w][threadIdx.x];
+        red64[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// One Gauss-Newton iteration at `level` for every session of the work list.
+__global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__restrict__ SP,
+                                                           const WorkItem *__restrict__ work, int level, int iter) {
+    __shared__ float T21s[16];
+    __shared__ double red64[kIcpVals];
+    __shared__ int is_last;
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    SessState *st = P.st;
+    if (!st->do_track || st->icp_failed || st->conv_level == level || iter >= P.iters[level] ||
+        (st->force_single && level != 0))
+        return;
+    if (threadIdx.x < 16) T21s[threadIdx.x] = st->T_21_f[threadIdx.x];
+    __syncthreads();
+    const LevelDesc &L = P.lev[level];
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    float A[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
+    bool ok = false;
+    if (x < L.W && y < L.H) ok = icp_summand(P, level, x, y, T21s, A, b);
+    float v[kIcpVals];
+    pixel_terms(ok, A, b, v);
+    block_terms(v, red64);
+    const int ntiles = L.tiles_x * L.tiles_y;
+    if (threadIdx.x < kIcpVals) P.icp_part[(size_t)wi.tile * kIcpVals + threadIdx.x] = red64[threadIdx.x];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = (atomicAdd(&st->icp_counter, 1u) == (unsigned)(ntiles - 1));
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    // Last workgroup of this session: FP64 sum of all partials in tile order.
+    double acc[kIcpVals];
+#pragma unroll
+    for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0;
+    for (int t = threadIdx.x; t < ntiles; t += kTileThreads) {
+        const double *pp = P.icp_part + (size_t)t * kIcpVals;
+#pragma unroll
+        for (int k = 0; k < kIcpVals; ++k) acc[k] += __builtin_nontemporal_load(pp + k);
+    }
+    __shared__ double wred[kTileThreads / 64][kIcpVals];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kIcpVals; ++k) {
+        double s = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) wred[wv][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double tot[kIcpVals];
+    for (int k = 0; k < kIcpVals; ++k) tot[k] = wred[0][k] + wred[1][k] + wred[2][k] + wred[3][k];
+    double ATA[36], ATb[6], x6[6];
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            ATA[i * 6 + j] = ATA[j * 6 + i] = tot[k++];
+        }
+    for (int i = 0; i < 6; ++i) ATb[i] = tot[k++];
+    const double cnt = tot[21 + 6];
+    for (int i = 0; i < 36; ++i) st->last_ATA[i] = ATA[i];
+    for (int i = 0; i < 6; ++i) st->last_ATb[i] = ATb[i];
+    st->last_valid = cnt;
+    st->solves += 1;
+    st->icp_counter = 0u;
+    if (P.icp_min_valid > 0 && cnt < (double)P.icp_min_valid) {
+        st->icp_failed = 7;
+        return;
+    }
+    if (!cholesky_solve(ATA, ATb, x6)) {
+        st->icp_failed = 4;
+        return;
+    }
+    double ninf = 0.0;
+    for (int i = 0; i < 6; ++i) ninf = fmax(ninf, fabs(x6[i]));
+    if (ninf < 5.0e-7) {
+        st->conv_level = level;
+        return;
+    }
+    double E[16];
+    expmap(x6, E);
+    matmul4d(E, st->T_12, st->T_12);
+    publish_T21(st);
+}
+
+// Per-frame / per-track reset of the tracker state (one thread per session).
+__global__ void k_track_setup(const SessParams *__restrict__ SP, int nsess, Mat16d Tinit, int has_init,
+                              int force_track) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsess) return;
+    SessState *st = SP[s].st;
+    st->do_track = force_track ? 1 : (st->frames > 0);
+    st->icp_failed = 0;
+    st->conv_level = -1;
+    st->solves = 0;
+    st->icp_counter = 0u;
+    st->force_single = has_init;
+    for (int k = 0; k < 16; ++k) st->T_12[k] = has_init ? Tinit.m[k] : ((k % 5) == 0 ? 1.0 : 0.0);
+    publish_T21(st);
+}
+
+__device__ void set_pose_derived(const SessParams &P, SessState *st) {
+    rigid_inverse(st->T_gc, st->T_cg);
+    for (int k = 0; k < 16; ++k) {
+        st->T_gc_f[k] = (float)st->T_gc[k];
+        st->T_cg_f[k] = (float)st->T_cg[k];
+    }
+    // raycast constants from the FP32 pose (d_gcc_Get_Camera_Center and
+    // rmi_Render_Maps_for_ICP, tsdf_renderer.cu:1667-1675)
+    const float *T = st->T_gc_f;
+    const float tx = T[3], ty = T[7], tz = T[11];
+    const f3 cam = {-T[0] * tx - T[4] * ty - T[8] * tz, -T[1] * tx - T[5] * ty - T[9] * tz,
+                    -T[2] * tx - T[6] * ty - T[10] * tz};
+    const f3 vcc = sub({P.cube_cen[0], P.cube_cen[1], P.cube_cen[2]}, cam);
+    const f3 ncc = normalised(vcc);
+    const float d = sqrtf(dot(vcc, vcc));
+    const float s = P.r_cube / d;
+    st->cam[0] = cam.x; st->cam[1] = cam.y; st->cam[2] = cam.z;
+    st->ncc[0] = ncc.x; st->ncc[1] = ncc.y; st->ncc[2] = ncc.z;
+    st->dist_cc = d;
+    st->cos_max = (s < 1.0f) ? sqrtf(1.0f - s * s) : -2.0f;
+}
+
+__device__ double rotation_deg(const double *T) {
+    double c = (T[0] + T[5] + T[10] - 1.0) * 0.5;
+    c = fmax(-1.0, fmin(1.0, c));
+    return acos(c) * 57.295779513082320876798;
+}
+
+// Apply the tracked increment (T_gc <- T_12 T_gc, pose_tracker.cu:992-998) or
+// record the failure; decides whether the frame is integrated.
+__global__ void k_track_finalize(const SessParams *__restrict__ SP, int nsess) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsess) return;
+    const SessParams &P = SP[s];
+    SessState *st = P.st;
+    int status = 0;
+    if (st->do_track) {
+        if (st->icp_failed) {
+            status = st->icp_failed;
+        } else {
+            const double *T = st->T_12;
+            const double tn = sqrt(T[3] * T[3] + T[7] * T[7] + T[11] * T[11]);
+            if ((P.max_motion_m > 0.0f && tn > P.max_motion_m) ||
+                (P.max_motion_deg > 0.0f && rotation_deg(T) > P.max_motion_deg)) {
+                status = 5;
+            } else {
+                matmul4d(st->T_12, st->T_gc, st->T_gc);
+                set_pose_derived(P, st);
+            }
+        }
+    }
+    st->last_status = status;
+    st->fuse_ok = (status == 0);
+    if (status == 0) st->frames += 1;
+}
+
+// Pose set from the host: derived copies only.
+__global__ void k_pose_derived(const SessParams *__restrict__ SP) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) set_pose_derived(SP[0], SP[0].st);
+}
+
+// Normal equations of one iteration without a solve (diagnostics/parity).
+__global__ __launch_bounds__(kTileThreads) void k_icp_system(const SessParams *__restrict__ SP,
+                                                             const WorkItem *__restrict__ work, int level, Mat16f T21,
+                                                             double *out_part, uint8_t *mask) {
+    __shared__ double red64[kIcpVals];
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const LevelDesc &L = P.lev[level];
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    float A[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
+    bool ok = false;
+    if (x < L.W && y < L.H) {
+        ok = icp_summand(P, level, x, y, T21.m, A, b);
+        if (mask) mask[(size_t)y * L.W + x] = ok ? 1 : 0;
+    }
+    float v[kIcpVals];
+    pixel_terms(ok, A, b, v);
+    block_terms(v, red64);
+    if (threadIdx.x < kIcpVals) out_part[(size_t)wi.tile * kIcpVals + threadIdx.x] = red64[threadIdx.x];
+}
+
+// Standalone integrate/raycast calls always run: mark the frame fusable.
+__global__ void k_fuse_ok(const SessParams *__restrict__ SP, int nsess) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsess) SP[s].st->fuse_ok = 1;
+}
+void launch_fuse_ok(const SessParams *P, int nsess, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse_ok, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess);
+}
+
+void launch_track_setup(const SessParams *P, int nsess, const double *T_init, hipStream_t s) {
+    Mat16d m{};
+    int has = T_init != nullptr;
+    if (has)
+        for (int k = 0; k < 16; ++k) m.m[k] = T_init[k];
+    // force_track: an explicit T_init call (vs_track) always tracks
+    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 0);
+}
+void launch_track_setup_forced(const SessParams *P, int nsess, const double *T_init, hipStream_t s) {
+    Mat16d m{};
+    int has = T_init != nullptr;
+    if (has)
+        for (int k = 0; k < 16; ++k) m.m[k] = T_init[k];
+    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 1);
+}
+void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int level, int iter, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_icp_iter, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
+}
+void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s) {
+    hipLaunchKernelGGL(k_track_finalize, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess);
+}
+void launch_pose_derived(const SessParams *P, hipStream_t s) {
+    hipLaunchKernelGGL(k_pose_derived, dim3(1), dim3(64), 0, s, P);
+}
+void launch_icp_system(const SessParams *P, const WorkItem *work, int n, int level, const float *T21,
+                       double *out_part, uint8_t *mask, hipStream_t s) {
+    Mat16f m;
+    for (int k = 0; k < 16; ++k) m.m[k] = T21[k];
+    if (n > 0)
+        hipLaunchKernelGGL(k_icp_system, dim3(n), dim3(kTileThreads), 0, s, P, work, level, m, out_part, mask);
+}
+
+}  // namespace vs

@@ -1,0 +1,149 @@
+// vs_img.hip — image stages of the fusion step: bilateral depth filter (A2-A3),
+// depth pyramid (A4) and the vertex/normal-map operator (A6-A7).
+//
+// Launch shape: one 256-thread workgroup (4 wave64) per 32 x 8 output tile,
+// tiles of every session of a batch concatenated in one work list, so a
+// single launch serves the whole batch. Rows of 32 pixels keep global loads
+// and stores coalesced (a wave covers two 128-byte rows of a float image).
+#include "vs_device.hpp"
+#include "vs_launch.hpp"
+
+namespace vs {
+
+constexpr int kHalo = 3;  // largest bilateral radius supported
+
+// Bilateral filter of the raw uint16 mm frame into pyramid level 0 (metres).
+// Reference: d_bilateral_filter (z_yooji_cuda_img_proc.cu:103-141): weight
+// g[i] g[j] exp(-(d - c)^2 / (2 sigma_r^2)), output 0 if any tap <= 0. Taps are
+// point samples with clamp-to-edge addressing; the sum is accumulated about
+// the centre value (c + sum w (d - c) / sum w), which is the same weighted mean
+// with a smaller rounding error.
+__global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__restrict__ SP,
+                                                            const WorkItem *__restrict__ work) {
+    __shared__ float tile[kTileH + 2 * kHalo][kTileW + 2 * kHalo];
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const LevelDesc &L = P.lev[0];
+    const int W = L.W, H = L.H;
+    const int x0 = (wi.tile % L.tiles_x) * kTileW, y0 = (wi.tile / L.tiles_x) * kTileH;
+    const int R = P.bil_radius;
+    const uint16_t *raw = P.raw;
+    constexpr int TW = kTileW + 2 * kHalo, TH = kTileH + 2 * kHalo;
+    for (int i = threadIdx.x; i < TW * TH; i += kTileThreads) {
+        int ly = i / TW, lx = i - ly * TW;
+        int gx = min(max(x0 + lx - kHalo, 0), W - 1);
+        int gy = min(max(y0 + ly - kHalo, 0), H - 1);
+        tile[ly][lx] = depth_m(raw[(size_t)gy * W + gx]);
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % kTileW, ly = threadIdx.x / kTileW;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= W || y >= H) return;
+    const float c = tile[ly + kHalo][lx + kHalo];
+    float s = 0.0f, t = 0.0f;
+    bool zero = false;
+    for (int i = -R; i <= R; ++i) {
+        const float gi = P.bil_g[i + R];
+        for (int j = -R; j <= R; ++j) {
+            const float p = tile[ly + kHalo + i][lx + kHalo + j];
+            zero |= (p <= 0.0f);
+            const float d = p - c;
+            const float w = gi * P.bil_g[j + R] * __expf(-(d * d) * P.bil_inv2s2);
+            t = fmaf(w, d, t);
+            s += w;
+        }
+    }
+    P.filt[0][(size_t)y * W + x] = zero ? 0.0f : c + t / s;
+}
+
+// One pyramid step, level l-1 -> l: the separable [1 4 6 4 1]^2/256 blur of
+// level l-1 sampled at (2i+1, 2j+1), 0 where the 5x5 window leaves the image.
+// Reference: cpdd_Construct_Pyramids_Down_Depth + sdg5 + ddh
+// (z_yooji_image_processor.cpp:368-407, 857-940, 567-596). Only the decimated
+// pixels are computed.
+__global__ __launch_bounds__(kTileThreads) void k_pyr_down(const SessParams *__restrict__ SP,
+                                                           const WorkItem *__restrict__ work, int l) {
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const LevelDesc &L = P.lev[l];
+    const int Wi = P.lev[l - 1].W, Hi = P.lev[l - 1].H;
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    if (x >= L.W || y >= L.H) return;
+    const int i = 2 * x + 1, j = 2 * y + 1;
+    float v = 0.0f;
+    if (i >= 2 && i <= Wi - 3 && j >= 2 && j <= Hi - 3) {
+        const float *in = P.filt[l - 1];
+        float h[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const float *row = in + (size_t)(j + r - 2) * Wi;
+            h[r] = 1.0f * row[i - 2] + 1.0f * row[i + 2] + 4.0f * row[i - 1] + 4.0f * row[i + 1] + 6.0f * row[i];
+        }
+        v = (1.0f * h[0] + 1.0f * h[4] + 4.0f * h[1] + 4.0f * h[3] + 6.0f * h[2]) / (16.0f * 16.0f);
+    }
+    P.filt[l][(size_t)y * L.W + x] = v;
+}
+
+// Vertex and normal maps of pyramid level l (AoS float3, -100 invalid).
+// Reference: cpm_Compute_Point_Map / enm_Estimate_Normal_Map
+// (z_yooji_image_processor.cpp:119-227): n = normalize(dz/dx, dz/dy, -1) from
+// central differences, 1-pixel border skipped, |dx| or |dy| < 1e-7 rejected,
+// and grazing normals (-n_z <= cos 60 deg) invalidate vertex and normal.
+// Evaluated on the input vertex map (order-independent form).
+__global__ __launch_bounds__(kTileThreads) void k_vnmap(const SessParams *__restrict__ SP,
+                                                        const WorkItem *__restrict__ work, int l, float *vtx,
+                                                        float *nrm) {
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const LevelDesc &L = P.lev[l];
+    const int W = L.W, H = L.H;
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    if (x >= W || y >= H) return;
+    const float *D = P.filt[l];
+    const float *K = L.K;
+    auto vert = [&](int xx, int yy, f3 &v) -> bool {
+        float d = D[(size_t)yy * W + xx];
+        if (d <= 0.0f) return false;
+        v = {d * ((float)xx - K[2]) / K[0], d * ((float)yy - K[3]) / K[1], d};
+        return true;
+    };
+    const size_t t = (size_t)y * W + x;
+    f3 c;
+    bool vc = vert(x, y, c);
+    float o[6] = {-100.f, -100.f, -100.f, -100.f, -100.f, -100.f};
+    if (vc) {
+        o[0] = c.x; o[1] = c.y; o[2] = c.z;
+    }
+    if (vc && x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
+        f3 x1, x2, y1, y2;
+        if (vert(x - 1, y, x1) && vert(x + 1, y, x2) && vert(x, y - 1, y1) && vert(x, y + 1, y2)) {
+            float dx = x2.x - x1.x, dy = y2.y - y1.y;
+            if (fabsf(dx) >= 1.0e-7f && fabsf(dy) >= 1.0e-7f) {
+                float gx = (x2.z - x1.z) / dx, gy = (y2.z - y1.z) / dy;
+                float nn = sqrtf(gx * gx + gy * gy + 1.0f);
+                float nx = gx / nn, ny = gy / nn, nz = -1.0f / nn;
+                if (-nz <= 0.5f) {
+                    o[0] = o[1] = o[2] = -100.f;
+                } else {
+                    o[3] = nx; o[4] = ny; o[5] = nz;
+                }
+            }
+        }
+    }
+    vtx[3 * t + 0] = o[0]; vtx[3 * t + 1] = o[1]; vtx[3 * t + 2] = o[2];
+    nrm[3 * t + 0] = o[3]; nrm[3 * t + 1] = o[4]; nrm[3 * t + 2] = o[5];
+}
+
+void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_bilateral, dim3(n), dim3(kTileThreads), 0, s, P, work);
+}
+void launch_pyr_down(const SessParams *P, const WorkItem *work, int n, int l, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_pyr_down, dim3(n), dim3(kTileThreads), 0, s, P, work, l);
+}
+void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int l, float *vtx, float *nrm, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vnmap, dim3(n), dim3(kTileThreads), 0, s, P, work, l, vtx, nrm);
+}
+
+}  // namespace vs

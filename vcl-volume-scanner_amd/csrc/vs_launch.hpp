@@ -1,0 +1,43 @@
+// vs_launch.hpp — host-side launchers of the fusion kernels. Every launcher
+// takes the device array of session parameters and a work list; n is the
+// number of work items (one workgroup each) unless stated otherwise.
+#pragma once
+
+#include "vs_device.hpp"
+
+namespace vs {
+
+// image stages (vs_img.hip)
+void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s);
+void launch_pyr_down(const SessParams *P, const WorkItem *work, int n, int level, hipStream_t s);
+void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int level, float *vtx, float *nrm, hipStream_t s);
+
+// tracker (vs_icp.hip); nsess = sessions in the batch
+void launch_track_setup(const SessParams *P, int nsess, const double *T_init, hipStream_t s);
+void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int level, int iter, hipStream_t s);
+void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s);
+// one system, no solve: writes 29 doubles (+ optional mask) for diagnostics
+void launch_icp_system(const SessParams *P, const WorkItem *work, int n, int level, const float *T21,
+                       double *out_part, uint8_t *mask, hipStream_t s);
+
+// volume stages (vs_vol.hip)
+struct BrickList {
+    int2 *items;        // (session, brick)
+    unsigned *count;    // device counter
+    unsigned capacity;
+};
+void launch_vis_clear(const SessParams *P, int nsess, BrickList bl, hipStream_t s);
+void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList bl, hipStream_t s);
+void launch_integrate(const SessParams *P, BrickList bl, int tsdf_type, int weight_type, int grid, hipStream_t s);
+void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
+                    hipStream_t s);
+// volume maintenance (single session, P points at its parameters)
+void launch_volume_init(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, uint32_t nwords,
+                        hipStream_t s);
+void launch_volume_to_linear(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, float *F, float *Wt,
+                             hipStream_t s);
+void launch_volume_from_linear(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, const float *F,
+                               const float *Wt, hipStream_t s);
+void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int weight_type, hipStream_t s);
+
+}  // namespace vs

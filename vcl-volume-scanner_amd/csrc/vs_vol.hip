@@ -1,0 +1,487 @@
+// vs_vol.hip — volume stages: brick visibility (A13), TSDF integration (A14),
+// raycast of the ICP model maps (A16-A18), and volume maintenance.
+//
+// Volume layout in HBM: brick-major. The cube is cut into 8^3 bricks; brick
+// (bx,by,bz) has index (bz*BH + by)*BW + bx and its 512 voxels are stored
+// contiguously, z-major then y then x inside the brick. A brick of FP32 values
+// is 2 KB, of FP16 values 1 KB, so one workgroup streams a whole brick with
+// fully coalesced accesses, and the 2x2x2 trilinear and 32-tap gradient
+// stencils of the raycast mostly stay inside one or two bricks.
+//
+// Per-session bitmaps: `bvis` (bricks touched by the current frame, rebuilt
+// each frame) and `bocc` (bricks holding any voxel of weight > 0; weights never
+// decrease, so the bit is set once and kept).
+#include "vs_device.hpp"
+#include "vs_launch.hpp"
+
+namespace vs {
+
+// ---------------------------------------------------------------------------
+// A13: visible bricks (g_fvsc_Find_Visible_Sub_Cubes, volume_integrator.cu:84-152)
+// ---------------------------------------------------------------------------
+__global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickList bl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *bl.count = 0u;
+    const int s = blockIdx.y;
+    if (s >= nsess) return;
+    const SessParams &P = SP[s];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.st->vis_count = 0u;
+        P.st->upd_count = 0ull;
+    }
+    if (!P.st->fuse_ok) return;
+    const uint32_t nb = (uint32_t)P.bdim[0] * P.bdim[1] * P.bdim[2];
+    const uint32_t nw = (nb + 31) / 32;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) P.bvis[w] = 0u;
+}
+
+// Marks the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
+// pixel's ray and appends each newly marked brick once to the batch's list.
+__global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__restrict__ SP,
+                                                           const WorkItem *__restrict__ work, BrickList bl) {
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const SessState *st = P.st;
+    if (!st->fuse_ok) return;
+    const LevelDesc &L = P.lev[0];
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    if (x >= L.W || y >= L.H) return;
+    const float td = depth_m(P.raw[(size_t)y * L.W + x]);
+    if (!(td > 0.0f)) return;
+    const float *Tcg = st->T_cg_f;
+    const f3 p = transform(Tcg, back_project((float)x, (float)y, L.K, td));
+    const f3 pe = transform(Tcg, back_project((float)x, (float)y, L.K, 100.0f));
+    const f3 dir = normalised(sub(pe, p));
+    for (int n = -4; n <= 1; ++n) {
+        const f3 q = add(p, scale((float)n * P.mu, dir));
+        f3 tv;
+        if (!world_to_voxel(P, q, tv)) continue;
+        // ROUNDF(v)/8 with the macro's precedence: the negative branch is not divided
+        const int bx = (tv.x < 0.0f) ? (int)(tv.x - 0.5f) : (int)(tv.x + 0.5f) / kBrick;
+        const int by = (tv.y < 0.0f) ? (int)(tv.y - 0.5f) : (int)(tv.y + 0.5f) / kBrick;
+        const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)(tv.z + 0.5f) / kBrick;
+        if (bx < 0 || bx >= P.bdim[0] || by < 0 || by >= P.bdim[1] || bz < 0 || bz >= P.bdim[2]) break;
+        const uint32_t b = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;
+        const uint32_t bit = 1u << (b & 31u);
+        uint32_t *word = P.bvis + (b >> 5);
+        if (__atomic_load_n(word, __ATOMIC_RELAXED) & bit) continue;
+        const uint32_t old = atomicOr(word, bit);
+        if (!(old & bit)) {
+            const unsigned idx = atomicAdd(bl.count, 1u);
+            if (idx < bl.capacity) bl.items[idx] = make_int2(wi.sess, (int)b);
+            atomicAdd(&P.st->vis_count, 1u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A14: TSDF update of the listed bricks (g_ctv_Compute_TSDF_of_Voxels,
+// volume_integrator.cu:154-262). Persistent: each workgroup walks the brick
+// list with a grid stride; 256 threads x 2 voxels = one brick per step.
+// Value and weight are loaded only for voxels that pass the projective test.
+// ---------------------------------------------------------------------------
+template <class TV, class TW>
+__global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__restrict__ SP, BrickList bl) {
+    const unsigned n = min(*bl.count, bl.capacity);
+    const int t = threadIdx.x;
+    const int xl = t & 7, yl = (t >> 3) & 7, zl = t >> 6;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const int2 it = bl.items[i];
+        const SessParams &P = SP[it.x];
+        const SessState *st = P.st;
+        const int b = it.y;
+        const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
+        TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
+        TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
+        const LevelDesc &L = P.lev[0];
+        const float *T = st->T_gc_f;
+        const float gx = ((float)(bx * kBrick + xl) + 0.5f) / P.vox_inv + P.origin[0];
+        const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
+        int nupd = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int zc = bz * kBrick + zl + 4 * h;
+            const float gz = ((float)zc + 0.5f) / P.vox_inv + P.origin[2];
+            const f3 p = transform(T, {gx, gy, gz});
+            float u, v;
+            project(p, L.K, u, v);
+            int iu, iv;
+            if (!roundf_ref(u, iu) || !roundf_ref(v, iv)) continue;
+            if (iu < 0 || iu > L.W - 1 || iv < 0 || iv > L.H - 1) continue;
+            const float dm = depth_m(P.raw[(size_t)iv * L.W + iu]);
+            if (!(dm > 0.0f)) continue;
+            const float td = dm - p.z;
+            if (td < -P.mu) continue;
+            const int li = t + kTileThreads * h;
+            const float tsd = fminf(1.0f, __fdividef(td, P.mu));
+            const float w = ld_w(Wt, li), f = ld_val(F, li);
+            const float nw = w + 1.0f;
+            st_val(F, li, __fdividef(fmaf(w, f, tsd), nw));
+            st_w(Wt, li, fminf(nw, P.max_w));
+            ++nupd;
+        }
+        const int c1 = __syncthreads_count(nupd >= 1), c2 = __syncthreads_count(nupd >= 2);
+        if (t == 0 && c1 > 0) {
+            atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
+            atomicAdd(&P.st->upd_count, (unsigned long long)(c1 + c2));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A16-A18: raycast (d_crtc_Cast_Ray_on_TSDF_Cube_NEW + d_csnt + g_rmi,
+// tsdf_renderer.cu:68-153, 288-635, 1069-1209). FP32 march with the
+// reference's step rules; taps in bricks whose occupancy bit is clear read as
+// "no data" (1.0) without touching the volume, which is exactly what the
+// reference's weight test returns there.
+// ---------------------------------------------------------------------------
+template <class TV, class TW>
+struct Vol {
+    const SessParams &P;
+    const TV *F;
+    const TW *Wt;
+    __device__ __forceinline__ bool tap(int x, int y, int z, float &out) const {
+        out = 1.0f;
+        if (x < 0 || x >= P.vdim[0] || y < 0 || y >= P.vdim[1] || z < 0 || z >= P.vdim[2]) return false;
+        const int b = brick_of(P, x, y, z);
+        if (!(P.bocc[b >> 5] & (1u << (b & 31)))) return false;
+        const size_t idx = (size_t)b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
+        if (!(ld_w(Wt, idx) > 0.0f)) return false;
+        out = ld_val(F, idx);
+        return true;
+    }
+    // d_gtvu: nearest voxel by ROUNDF
+    __device__ __forceinline__ bool nearest(f3 v, float &out) const {
+        int x, y, z;
+        out = 1.0f;
+        if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z)) return false;
+        return tap(x, y, z, out);
+    }
+    // d_gtvi: trilinear with toInt = truncation; any invalid tap -> false, 1.0
+    __device__ __forceinline__ bool trilinear(f3 v, float &out) const {
+        out = 1.0f;
+        if (!(fabsf(v.x) < 2147483520.0f && fabsf(v.y) < 2147483520.0f && fabsf(v.z) < 2147483520.0f)) return false;
+        const int ox = (int)v.x, oy = (int)v.y, oz = (int)v.z;
+        const float rx = v.x - (float)ox, ry = v.y - (float)oy, rz = v.z - (float)oz;
+        float f0, f1, f2, f3v, b0, b1, b2, b3;
+        if (!tap(ox, oy, oz, f0) || !tap(ox + 1, oy, oz, f1) || !tap(ox, oy + 1, oz, f2) ||
+            !tap(ox + 1, oy + 1, oz, f3v))
+            return false;
+        if (!tap(ox, oy, oz + 1, b0) || !tap(ox + 1, oy, oz + 1, b1) || !tap(ox, oy + 1, oz + 1, b2) ||
+            !tap(ox + 1, oy + 1, oz + 1, b3))
+            return false;
+        const float fi = (1.0f - ry) * ((1.0f - rx) * f0 + rx * f1) + ry * ((1.0f - rx) * f2 + rx * f3v);
+        const float bi = (1.0f - ry) * ((1.0f - rx) * b0 + rx * b1) + ry * ((1.0f - rx) * b2 + rx * b3);
+        out = (1.0f - rz) * fi + rz * bi;
+        return true;
+    }
+};
+
+struct RayConst {
+    float K[4];
+    f3 cam, ncc;
+    float dist_cc, cos_max, r_cube, step_scale, total_max;
+    int max_iter;
+};
+
+template <class TV, class TW, bool STATS>
+__device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C, const float *Tcg, float px,
+                                         float py, f3 &hit, unsigned &n_near, unsigned &n_tri) {
+    const float dist_min = C.dist_cc - C.r_cube, dist_max = C.dist_cc + C.r_cube;
+    const f3 pe = transform(Tcg, back_project(px, py, C.K, dist_max));
+    const f3 rd = normalised(sub(pe, C.cam));
+    if (dot(rd, C.ncc) < C.cos_max) return false;
+    const f3 ps = add(C.cam, scale(dist_min, rd));
+    f3 vox;
+    world_to_voxel(V.P, ps, vox);
+    float tsdf;
+    if (STATS) ++n_near;
+    const bool valid = V.nearest(vox, tsdf);
+    bool coarse;
+    if (!valid) coarse = true;
+    else if (tsdf >= 0.0f) coarse = false;
+    else return false;
+    bool behind = false;
+    float total = 0.0f;
+    int iter = 0;
+    while (true) {
+        if (++iter > C.max_iter) return false;
+        const float step = coarse ? C.step_scale
+                                  : ((tsdf > 0.0f) ? fmaxf(C.step_scale * tsdf, 1.0f) : fminf(C.step_scale * tsdf, -1.0f));
+        vox = add(vox, scale(step, rd));
+        total += step;
+        if (total > C.total_max) return false;
+        if (STATS) ++n_near;
+        V.nearest(vox, tsdf);
+        if (tsdf > -1.0f && tsdf < 1.0f) {
+            if (STATS) ++n_tri;
+            V.trilinear(vox, tsdf);
+            if (tsdf < 0.0f) {
+                if (coarse) return false;
+                behind = true;
+                break;
+            } else if (tsdf == 0.0f) {
+                hit = voxel_to_world(V.P, vox);
+                return true;
+            } else if (coarse) {
+                coarse = false;
+            }
+        }
+    }
+    (void)behind;
+    float step = C.step_scale * tsdf;
+    vox = add(vox, scale(step, rd));
+    if (STATS) ++n_tri;
+    V.trilinear(vox, tsdf);
+    step = C.step_scale * tsdf;
+    vox = add(vox, scale(step, rd));
+    hit = voxel_to_world(V.P, vox);
+    return true;
+}
+
+// d_csnt_Compute_Surface_Normal_from_TSDF: central differences of trilinear
+// values from 32 taps around the hit.
+template <class TV, class TW>
+__device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n) {
+    f3 vox;
+    if (!world_to_voxel(V.P, p, vox)) return false;
+    const int ox = (int)vox.x, oy = (int)vox.y, oz = (int)vox.z;
+    const float rx = vox.x - (float)ox, ry = vox.y - (float)oy, rz = vox.z - (float)oz;
+    // front/back slices: 12 taps (dx2,dy2); foremost/backmost: 4 taps (dx1,dy1)
+    const int dx1[4] = {0, 1, 0, 1}, dy1[4] = {0, 0, 1, 1};
+    const int dx2[12] = {0, 1, -1, 0, 1, 2, -1, 0, 1, 2, 0, 1};
+    const int dy2[12] = {-1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2};
+    float fm[4], fr[12], bk[12], bm[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (!V.tap(ox + dx1[i], oy + dy1[i], oz - 1, fm[i])) return false;
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        if (!V.tap(ox + dx2[i], oy + dy2[i], oz, fr[i])) return false;
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        if (!V.tap(ox + dx2[i], oy + dy2[i], oz + 1, bk[i])) return false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (!V.tap(ox + dx1[i], oy + dy1[i], oz + 2, bm[i])) return false;
+    auto Lz = [&](int i) { return (1.0f - rz) * fr[i] + rz * bk[i]; };
+    float a4[4], b4[4], a2[2], b2[2];
+    a4[0] = Lz(2); a4[1] = Lz(3); a4[2] = Lz(4); a4[3] = Lz(5);
+    b4[0] = Lz(6); b4[1] = Lz(7); b4[2] = Lz(8); b4[3] = Lz(9);
+    a2[0] = (1.0f - rx) * a4[0] + rx * a4[1];
+    a2[1] = (1.0f - rx) * b4[0] + rx * b4[1];
+    b2[0] = (1.0f - rx) * a4[2] + rx * a4[3];
+    b2[1] = (1.0f - rx) * b4[2] + rx * b4[3];
+    float gx = (1.0f - ry) * b2[0] + ry * b2[1];
+    gx -= (1.0f - ry) * a2[0] + ry * a2[1];
+    a4[0] = Lz(0); a4[1] = Lz(3); a4[2] = Lz(7); a4[3] = Lz(10);
+    b4[0] = Lz(1); b4[1] = Lz(4); b4[2] = Lz(8); b4[3] = Lz(11);
+    a2[0] = (1.0f - ry) * a4[0] + ry * a4[1];
+    a2[1] = (1.0f - ry) * b4[0] + ry * b4[1];
+    b2[0] = (1.0f - ry) * a4[2] + ry * a4[3];
+    b2[1] = (1.0f - ry) * b4[2] + ry * b4[3];
+    float gy = (1.0f - rx) * b2[0] + rx * b2[1];
+    gy -= (1.0f - rx) * a2[0] + rx * a2[1];
+    a4[0] = (1.0f - ry) * fm[0] + ry * fm[2];
+    a4[1] = (1.0f - ry) * fr[3] + ry * fr[7];
+    a4[2] = (1.0f - ry) * bk[3] + ry * bk[7];
+    a4[3] = (1.0f - ry) * bm[0] + ry * bm[2];
+    b4[0] = (1.0f - ry) * fm[1] + ry * fm[3];
+    b4[1] = (1.0f - ry) * fr[4] + ry * fr[8];
+    b4[2] = (1.0f - ry) * bk[4] + ry * bk[8];
+    b4[3] = (1.0f - ry) * bm[1] + ry * bm[3];
+    a2[0] = (1.0f - rz) * a4[0] + rz * a4[1];
+    a2[1] = (1.0f - rz) * b4[0] + rz * b4[1];
+    b2[0] = (1.0f - rz) * a4[2] + rz * a4[3];
+    b2[1] = (1.0f - rz) * b4[2] + rz * b4[3];
+    float gz = (1.0f - rx) * b2[0] + rx * b2[1];
+    gz -= (1.0f - rx) * a2[0] + rx * a2[1];
+    if (gx == 0.0f && gy == 0.0f && gz == 0.0f) return false;
+    n = normalised({gx, gy, gz});
+    return true;
+}
+
+template <class TV, class TW, bool STATS>
+__global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__restrict__ SP,
+                                                          const WorkItem *__restrict__ work, int level,
+                                                          unsigned long long *stats) {
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const SessState *st = P.st;
+    if (!st->fuse_ok) return;
+    const LevelDesc &L = P.lev[level];
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    if (x >= L.W || y >= L.H) return;
+    RayConst C;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) C.K[k] = L.K[k];
+    C.cam = {st->cam[0], st->cam[1], st->cam[2]};
+    C.ncc = {st->ncc[0], st->ncc[1], st->ncc[2]};
+    C.dist_cc = st->dist_cc;
+    C.cos_max = st->cos_max;
+    C.r_cube = P.r_cube;
+    C.step_scale = P.step_scale;
+    C.total_max = P.total_max;
+    C.max_iter = P.max_iter;
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight};
+    unsigned n_near = 0, n_tri = 0;
+    f3 p, n;
+    const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri) &&
+                    surface_normal(V, p, n);
+    const size_t t = (size_t)y * L.W + x;
+    if (ok) {
+        const float *T = st->T_gc_f;
+        const f3 pc = transform(T, p);
+        const f3 nc = {T[0] * n.x + T[1] * n.y + T[2] * n.z, T[4] * n.x + T[5] * n.y + T[6] * n.z,
+                       T[8] * n.x + T[9] * n.y + T[10] * n.z};
+        P.mvtx[level][t] = make_float4(pc.x, pc.y, pc.z, 0.0f);
+        P.mnrm[level][t] = make_float4(nc.x, nc.y, nc.z, 0.0f);
+        // light (0,0,-1): n.l = 0*nx + 0*ny - nz (z_yooji_object_scanner.cpp:1313)
+        const float sa = nc.x * 0.0f + nc.y * 0.0f + nc.z * -1.0f;
+        P.mimg[level][t] = (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f)));
+    } else {
+        P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
+        P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
+        P.mimg[level][t] = 0;
+    }
+    if (STATS) {
+        atomicAdd(stats + 0, (unsigned long long)n_near);
+        atomicAdd(stats + 1, (unsigned long long)n_tri);
+        if (ok) atomicAdd(stats + 2, 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// volume maintenance
+// ---------------------------------------------------------------------------
+template <class TV, class TW>
+__global__ void k_volume_init(const SessParams *__restrict__ SP, size_t nvox, uint32_t nwords) {
+    const SessParams &P = SP[0];
+    TV *F = (TV *)P.tsdf;
+    TW *Wt = (TW *)P.weight;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
+        st_val(F, i, 1.0f);
+        st_w(Wt, i, 0.0f);
+    }
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (size_t)gridDim.x * blockDim.x) {
+        P.bvis[w] = 0u;
+        P.bocc[w] = 0u;
+    }
+}
+
+// brick-major -> linear x-fastest, converted to float
+template <class TV, class TW>
+__global__ void k_volume_to_linear(const SessParams *__restrict__ SP, size_t nvox, float *F, float *Wt) {
+    const SessParams &P = SP[0];
+    const size_t WH = (size_t)P.vdim[0] * P.vdim[1];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i / WH), y = (int)((i / P.vdim[0]) % P.vdim[1]), x = (int)(i % P.vdim[0]);
+        const size_t j = vox_index(P, x, y, z);
+        if (F) F[i] = ld_val((const TV *)P.tsdf, j);
+        if (Wt) Wt[i] = ld_w((const TW *)P.weight, j);
+    }
+}
+template <class TV, class TW>
+__global__ void k_volume_from_linear(const SessParams *__restrict__ SP, size_t nvox, const float *F,
+                                     const float *Wt) {
+    const SessParams &P = SP[0];
+    const size_t WH = (size_t)P.vdim[0] * P.vdim[1];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i / WH), y = (int)((i / P.vdim[0]) % P.vdim[1]), x = (int)(i % P.vdim[0]);
+        const size_t j = vox_index(P, x, y, z);
+        st_val((TV *)P.tsdf, j, F[i]);
+        st_w((TW *)P.weight, j, Wt[i]);
+    }
+}
+// bocc from the weights: one workgroup per brick
+template <class TW>
+__global__ __launch_bounds__(kTileThreads) void k_occupancy(const SessParams *__restrict__ SP, uint32_t nbricks) {
+    const SessParams &P = SP[0];
+    for (uint32_t b = blockIdx.x; b < nbricks; b += gridDim.x) {
+        const TW *Wt = (const TW *)P.weight + (size_t)b * kBrickVox;
+        const int any = __syncthreads_or((ld_w(Wt, threadIdx.x) > 0.0f) || (ld_w(Wt, threadIdx.x + 256) > 0.0f));
+        if (threadIdx.x == 0) {
+            if (any) atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
+            else atomicAnd(P.bocc + (b >> 5), ~(1u << (b & 31)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_vis_clear(const SessParams *P, int nsess, BrickList bl, hipStream_t s) {
+    hipLaunchKernelGGL(k_vis_clear, dim3(8, nsess), dim3(256), 0, s, P, nsess, bl);
+}
+void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList bl, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_vis_mark, dim3(n), dim3(kTileThreads), 0, s, P, work, bl);
+}
+
+#define VS_DISPATCH(tt, wt, FN, ...)                                                \
+    do {                                                                            \
+        if ((tt) == 0 && (wt) == 0) FN<float, float>(__VA_ARGS__);                  \
+        else if ((tt) == 0 && (wt) == 1) FN<float, uint8_t>(__VA_ARGS__);           \
+        else if ((tt) == 1 && (wt) == 0) FN<__half, float>(__VA_ARGS__);            \
+        else FN<__half, uint8_t>(__VA_ARGS__);                                      \
+    } while (0)
+
+template <class TV, class TW>
+static void integrate_t(const SessParams *P, BrickList bl, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_integrate<TV, TW>), dim3(grid), dim3(kTileThreads), 0, s, P, bl);
+}
+void launch_integrate(const SessParams *P, BrickList bl, int tt, int wt, int grid, hipStream_t s) {
+    VS_DISPATCH(tt, wt, integrate_t, P, bl, grid, s);
+}
+
+template <class TV, class TW>
+static void raycast_t(const SessParams *P, const WorkItem *work, int n, int level, unsigned long long *stats,
+                      hipStream_t s) {
+    if (n <= 0) return;
+    if (stats)
+        hipLaunchKernelGGL((k_raycast<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, work, level, stats);
+    else
+        hipLaunchKernelGGL((k_raycast<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, work, level, stats);
+}
+void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
+    VS_DISPATCH(tt, wt, raycast_t, P, work, n, level, nullptr, s);
+}
+void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
+                          unsigned long long *stats, hipStream_t s) {
+    VS_DISPATCH(tt, wt, raycast_t, P, work, n, level, stats, s);
+}
+
+static int grid_for(size_t n) {
+    size_t g = (n + 255) / 256;
+    return (int)(g > 8192 ? 8192 : (g == 0 ? 1 : g));
+}
+template <class TV, class TW>
+static void init_t(const SessParams *P, size_t nvox, uint32_t nwords, hipStream_t s) {
+    hipLaunchKernelGGL((k_volume_init<TV, TW>), dim3(grid_for(nvox)), dim3(256), 0, s, P, nvox, nwords);
+}
+void launch_volume_init(const SessParams *P, size_t nvox, int tt, int wt, uint32_t nwords, hipStream_t s) {
+    VS_DISPATCH(tt, wt, init_t, P, nvox, nwords, s);
+}
+template <class TV, class TW>
+static void to_lin_t(const SessParams *P, size_t nvox, float *F, float *Wt, hipStream_t s) {
+    hipLaunchKernelGGL((k_volume_to_linear<TV, TW>), dim3(grid_for(nvox)), dim3(256), 0, s, P, nvox, F, Wt);
+}
+void launch_volume_to_linear(const SessParams *P, size_t nvox, int tt, int wt, float *F, float *Wt, hipStream_t s) {
+    VS_DISPATCH(tt, wt, to_lin_t, P, nvox, F, Wt, s);
+}
+template <class TV, class TW>
+static void from_lin_t(const SessParams *P, size_t nvox, const float *F, const float *Wt, hipStream_t s) {
+    hipLaunchKernelGGL((k_volume_from_linear<TV, TW>), dim3(grid_for(nvox)), dim3(256), 0, s, P, nvox, F, Wt);
+}
+void launch_volume_from_linear(const SessParams *P, size_t nvox, int tt, int wt, const float *F, const float *Wt,
+                               hipStream_t s) {
+    VS_DISPATCH(tt, wt, from_lin_t, P, nvox, F, Wt, s);
+}
+void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int wt, hipStream_t s) {
+    int g = (int)(nbricks > 16384 ? 16384 : nbricks);
+    if (g <= 0) return;
+    if (wt == 0) hipLaunchKernelGGL((k_occupancy<float>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
+    else hipLaunchKernelGGL((k_occupancy<uint8_t>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
+}
+
+}  // namespace vs

@@ -1,0 +1,397 @@
+"""ctypes binding of the fusion library's C ABI (include/vscan.h) and of the
+synthetic depth-stream generator (include/vscan_scene.h), plus the workload
+presets the tests and the benchmark share.
+
+The product is libvscan.so (HIP kernels for gfx950 behind the C ABI). This
+module only marshals arguments; it never computes a stage itself, and it
+raises if the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvscan.so")
+SCENE_PATH = os.path.join(HERE, "libvscan_scene.so")
+
+VS_OK, VS_EINVAL, VS_ENOMEM, VS_EHIP, VS_TRACK_LOST, VS_MOTION_REJECTED, VS_BUDGET, VS_TOO_FEW_POINTS = range(8)
+VS_TSDF_F32, VS_TSDF_F16 = 0, 1
+VS_WEIGHT_F32, VS_WEIGHT_U8 = 0, 1
+
+
+class SessionDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+        ("levels", C.c_int), ("iters", C.c_int * 3),
+        ("icp_dist2", C.c_float), ("icp_min_valid", C.c_int),
+        ("max_motion_m", C.c_float), ("max_motion_deg", C.c_float),
+        ("vol_dim", C.c_int * 3), ("origin", C.c_float * 3),
+        ("voxel", C.c_float), ("mu", C.c_float), ("max_weight", C.c_float),
+        ("tsdf_type", C.c_int), ("weight_type", C.c_int),
+        ("bil_sigma_s", C.c_float), ("bil_sigma_r", C.c_float), ("bil_radius", C.c_int),
+    ]
+
+
+class Prim(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_double * 6)]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+        ("n_prims", C.c_int), ("prims", Prim * 64),
+        ("radius0", C.c_double), ("radius1", C.c_double), ("height0", C.c_double), ("height_amp", C.c_double),
+        ("deg_per_frame", C.c_double), ("phase0_deg", C.c_double),
+        ("target", C.c_double * 3), ("n_frames", C.c_int),
+        ("seed", C.c_uint64), ("dropout", C.c_double), ("noise", C.c_int), ("max_range", C.c_double),
+    ]
+
+
+_lib = None
+_scene = None
+
+P = C.c_void_p
+dptr = C.POINTER(C.c_double)
+fptr = C.POINTER(C.c_float)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def lib():
+    """The fusion library; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run vcl-volume-scanner_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.vs_status_string.restype = C.c_char_p
+        L.vs_last_error.restype = C.c_char_p
+        L.vs_session_bytes.restype = C.c_size_t
+        L.vs_session_bytes.argtypes = [C.POINTER(SessionDesc)]
+        L.vs_session_desc_default.argtypes = [C.POINTER(SessionDesc), C.c_int, C.c_int, C.c_int, C.c_float]
+        L.vs_session_desc_default.restype = None
+        for name in ["vs_ctx_create", "vs_ctx_destroy", "vs_session_create", "vs_session_destroy",
+                     "vs_session_reset", "vs_session_desc_get", "vs_session_sync", "vs_set_depth", "vs_bilateral",
+                     "vs_pyramid", "vs_vertex_normal_maps", "vs_track", "vs_integrate", "vs_raycast",
+                     "vs_fuse_frame", "vs_fuse_batch", "vs_get_frame_status", "vs_get_pose", "vs_set_pose",
+                     "vs_get_filtered", "vs_set_filtered", "vs_get_maps", "vs_set_maps", "vs_volume_download",
+                     "vs_volume_upload", "vs_icp_system", "vs_session_set_timing", "vs_get_stage_ms",
+                     "vs_get_integrate_counts", "vs_raycast_counts"]:
+            getattr(L, name).restype = C.c_int
+        L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
+        L.vs_session_reset.argtypes = [P, P, P]
+        L.vs_set_depth.argtypes = [P, P, C.c_int, P]
+        L.vs_bilateral.argtypes = [P, P]
+        L.vs_pyramid.argtypes = [P, P]
+        L.vs_vertex_normal_maps.argtypes = [P, C.c_int, P, P, P]
+        L.vs_track.argtypes = [P, P, P]
+        L.vs_integrate.argtypes = [P, P]
+        L.vs_raycast.argtypes = [P, C.c_int, P]
+        L.vs_fuse_frame.argtypes = [P, P, C.c_int, P]
+        L.vs_fuse_batch.argtypes = [P, P, P, C.c_int, C.c_int, P]
+        L.vs_get_frame_status.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.vs_get_pose.argtypes = [P, P]
+        L.vs_set_pose.argtypes = [P, P, P]
+        L.vs_get_filtered.argtypes = [P, C.c_int, P]
+        L.vs_set_filtered.argtypes = [P, C.c_int, P]
+        L.vs_get_maps.argtypes = [P, C.c_int, P, P, P, P]
+        L.vs_set_maps.argtypes = [P, C.c_int, P, P]
+        L.vs_volume_download.argtypes = [P, P, P, P]
+        L.vs_volume_upload.argtypes = [P, P, P]
+        L.vs_icp_system.argtypes = [P, C.c_int, P, P, P, C.POINTER(C.c_longlong), P]
+        L.vs_session_set_timing.argtypes = [P, C.c_int]
+        L.vs_get_stage_ms.argtypes = [P, P]
+        L.vs_get_integrate_counts.argtypes = [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.vs_raycast_counts.argtypes = [P, C.c_int, P]
+        _lib = L
+    return _lib
+
+
+def scene_lib():
+    global _scene
+    if _scene is None:
+        if not os.path.exists(SCENE_PATH):
+            raise RuntimeError(f"{SCENE_PATH} is not built: run vcl-volume-scanner_amd/build.py")
+        S = C.CDLL(SCENE_PATH)
+        S.vs_scene_preset.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.POINTER(Scene)]
+        S.vs_scene_pose.argtypes = [C.POINTER(Scene), C.c_int, P]
+        S.vs_scene_render.argtypes = [C.POINTER(Scene), C.c_int, P]
+        S.vs_scene_render_pose.argtypes = [C.POINTER(Scene), P, C.c_int, P]
+        S.vs_scene_render_exact.argtypes = [C.POINTER(Scene), P, P]
+        S.vs_scene_sdf.argtypes = [C.POINTER(Scene), P]
+        S.vs_scene_sdf.restype = C.c_double
+        for f in ["vs_scene_preset", "vs_scene_pose", "vs_scene_render", "vs_scene_render_pose",
+                  "vs_scene_render_exact"]:
+            getattr(S, f).restype = None
+        _scene = S
+    return _scene
+
+
+class VsError(RuntimeError):
+    def __init__(self, code, where):
+        L = lib()
+        super().__init__(f"{where}: {L.vs_status_string(code).decode()} ({code}): {L.vs_last_error().decode()}")
+        self.code = code
+
+
+def check(code, where):
+    if code != VS_OK:
+        raise VsError(code, where)
+
+
+# ---------------------------------------------------------------------------
+# synthetic scenes
+# ---------------------------------------------------------------------------
+class SceneStream:
+    """Seeded depth stream: uint16 mm frames and ground-truth poses."""
+
+    def __init__(self, config, seed=1, width=0, height=0, noise=True, dropout=None):
+        self.sc = Scene()
+        scene_lib().vs_scene_preset(config, seed, width, height, C.byref(self.sc))
+        if not noise:
+            self.sc.noise = 0
+        if dropout is not None:
+            self.sc.dropout = dropout
+        self.W, self.H = self.sc.width, self.sc.height
+
+    @property
+    def K(self):
+        return (self.sc.fx, self.sc.fy, self.sc.cx, self.sc.cy)
+
+    def pose(self, frame):
+        T = np.zeros(16, np.float64)
+        scene_lib().vs_scene_pose(C.byref(self.sc), frame, _ptr(T))
+        return T.reshape(4, 4)
+
+    def frame(self, frame):
+        d = np.zeros((self.H, self.W), np.uint16)
+        scene_lib().vs_scene_render(C.byref(self.sc), frame, _ptr(d))
+        return d
+
+    def frame_at(self, T, frame=0):
+        d = np.zeros((self.H, self.W), np.uint16)
+        T = np.ascontiguousarray(T, np.float64)
+        scene_lib().vs_scene_render_pose(C.byref(self.sc), _ptr(T), frame, _ptr(d))
+        return d
+
+    def exact_depth(self, T):
+        d = np.zeros((self.H, self.W), np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        scene_lib().vs_scene_render_exact(C.byref(self.sc), _ptr(T), _ptr(d))
+        return d
+
+    def sdf(self, pts):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.empty(len(pts))
+        S = scene_lib()
+        for i, p in enumerate(pts):
+            out[i] = S.vs_scene_sdf(C.byref(self.sc), _ptr(p))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# workloads (the five configurations the measurements are quoted on)
+# ---------------------------------------------------------------------------
+WORKLOADS = {
+    # plain, CPU-runnable: 30 x 320x240, 128^3 FP32 over a 1 m cube, mu = 3 voxels
+    "plain_320x240_128": dict(config=0, W=320, H=240, n=128, cube=1.0, origin=(-0.5, -0.5, -0.3), mu_vox=3.0,
+                              tsdf=VS_TSDF_F32, weight=VS_WEIGHT_F32, max_w=100.0, frames=30, iters=(10, 5, 4)),
+    # plain on GPU: 640x480, 256^3 FP32 over 1.2 m, helical orbit, 300 frames
+    "plain_640x480_256": dict(config=1, W=640, H=480, n=256, cube=1.2, origin=(-0.6, -0.6, -0.35), mu_vox=3.0,
+                              tsdf=VS_TSDF_F32, weight=VS_WEIGHT_F32, max_w=100.0, frames=300, iters=(10, 5, 4)),
+    # precision form: 640x480, 512^3 FP16 + u8 (cap 128) over 1.5 m, 20 random primitives
+    "fp16_640x480_512": dict(config=2, W=640, H=480, n=512, cube=1.5, origin=(-0.75, -0.75, -0.4), mu_vox=3.0,
+                             tsdf=VS_TSDF_F16, weight=VS_WEIGHT_U8, max_w=128.0, frames=300, iters=(10, 5, 4)),
+}
+
+
+def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None):
+    d = SessionDesc()
+    W = W or w["W"]
+    H = H or w["H"]
+    n = n or w["n"]
+    lib().vs_session_desc_default(C.byref(d), W, H, n, w["cube"])
+    if K is not None:
+        d.fx, d.fy, d.cx, d.cy = K
+    vox = w["cube"] / n
+    d.voxel = vox
+    d.mu = w["mu_vox"] * vox
+    for k in range(3):
+        d.origin[k] = w["origin"][k]
+    d.max_weight = w["max_w"]
+    d.tsdf_type = w["tsdf"] if tsdf is None else tsdf
+    d.weight_type = w["weight"] if weight is None else weight
+    for k in range(3):
+        d.iters[k] = w["iters"][k]
+    d.bil_sigma_s = 4.0
+    d.bil_sigma_r = 0.030
+    d.bil_radius = 2
+    d.icp_dist2 = (0.055 * w["cube"]) ** 2
+    return d
+
+
+# ---------------------------------------------------------------------------
+# handles
+# ---------------------------------------------------------------------------
+class Context:
+    def __init__(self, device=0):
+        self.h = P()
+        check(lib().vs_ctx_create(device, C.byref(self.h)), "vs_ctx_create")
+
+    def close(self):
+        if self.h:
+            lib().vs_ctx_destroy(self.h)
+            self.h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def fuse_batch(self, sessions, depths, is_host=True, stream=None):
+        n = len(sessions)
+        hs = (P * n)(*[s.h for s in sessions])
+        keep = [np.ascontiguousarray(d, np.uint16) if is_host else d for d in depths]
+        ds = (P * n)(*[(_ptr(k) if is_host else P(k)) for k in keep])
+        check(lib().vs_fuse_batch(self.h, hs, ds, n, 1 if is_host else 0, stream), "vs_fuse_batch")
+
+
+class Session:
+    def __init__(self, ctx, desc, T_gc0=None):
+        self.ctx = ctx
+        self.desc = desc
+        self.W, self.H = desc.width, desc.height
+        self.h = P()
+        T = None if T_gc0 is None else np.ascontiguousarray(T_gc0, np.float64).reshape(16)
+        check(lib().vs_session_create(ctx.h, C.byref(desc), _ptr(T), C.byref(self.h)), "vs_session_create")
+
+    def close(self):
+        if self.h:
+            lib().vs_session_destroy(self.h)
+            self.h = P()
+
+    def level_shape(self, l):
+        return (self.H >> l, self.W >> l)
+
+    @property
+    def nvox(self):
+        d = self.desc
+        return d.vol_dim[0] * d.vol_dim[1] * d.vol_dim[2]
+
+    def set_depth(self, depth):
+        depth = np.ascontiguousarray(depth, np.uint16)
+        check(lib().vs_set_depth(self.h, _ptr(depth), 1, None), "vs_set_depth")
+
+    def bilateral(self):
+        check(lib().vs_bilateral(self.h, None), "vs_bilateral")
+
+    def pyramid(self):
+        check(lib().vs_pyramid(self.h, None), "vs_pyramid")
+
+    def track(self, T_init=None):
+        T = None if T_init is None else np.ascontiguousarray(T_init, np.float64).reshape(16)
+        check(lib().vs_track(self.h, _ptr(T), None), "vs_track")
+
+    def integrate(self):
+        check(lib().vs_integrate(self.h, None), "vs_integrate")
+
+    def raycast(self, level):
+        check(lib().vs_raycast(self.h, level, None), "vs_raycast")
+
+    def fuse(self, depth):
+        depth = np.ascontiguousarray(depth, np.uint16)
+        check(lib().vs_fuse_frame(self.h, _ptr(depth), 1, None), "vs_fuse_frame")
+
+    def sync(self):
+        check(lib().vs_session_sync(self.h), "vs_session_sync")
+
+    def status(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().vs_get_frame_status(self.h, C.byref(a), C.byref(b), C.byref(c)), "vs_get_frame_status")
+        return a.value, b.value, c.value
+
+    def pose(self):
+        T = np.zeros(16, np.float64)
+        check(lib().vs_get_pose(self.h, _ptr(T)), "vs_get_pose")
+        return T.reshape(4, 4)
+
+    def set_pose(self, T):
+        T = np.ascontiguousarray(T, np.float64).reshape(16)
+        check(lib().vs_set_pose(self.h, _ptr(T), None), "vs_set_pose")
+
+    def filtered(self, level):
+        out = np.zeros(self.level_shape(level), np.float32)
+        check(lib().vs_get_filtered(self.h, level, _ptr(out)), "vs_get_filtered")
+        return out
+
+    def set_filtered(self, level, img):
+        img = np.ascontiguousarray(img, np.float32)
+        check(lib().vs_set_filtered(self.h, level, _ptr(img)), "vs_set_filtered")
+
+    def maps(self, level):
+        h, w = self.level_shape(level)
+        d = np.zeros((h, w), np.float32)
+        v = np.zeros((h, w, 3), np.float32)
+        n = np.zeros((h, w, 3), np.float32)
+        img = np.zeros((h, w), np.uint8)
+        check(lib().vs_get_maps(self.h, level, _ptr(d), _ptr(v), _ptr(n), _ptr(img)), "vs_get_maps")
+        return d, v, n, img
+
+    def set_maps(self, level, vtx, nrm):
+        vtx = np.ascontiguousarray(vtx, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        check(lib().vs_set_maps(self.h, level, _ptr(vtx), _ptr(nrm)), "vs_set_maps")
+
+    def volume(self):
+        d = self.desc
+        shp = (d.vol_dim[2], d.vol_dim[1], d.vol_dim[0])
+        F = np.zeros(shp, np.float32)
+        Wt = np.zeros(shp, np.float32)
+        nb = (d.vol_dim[0] // 8) * (d.vol_dim[1] // 8) * (d.vol_dim[2] // 8)
+        bm = np.zeros(nb, np.uint8)
+        check(lib().vs_volume_download(self.h, _ptr(F), _ptr(Wt), _ptr(bm)), "vs_volume_download")
+        return F, Wt, bm
+
+    def set_volume(self, F, Wt):
+        F = np.ascontiguousarray(F, np.float32)
+        Wt = np.ascontiguousarray(Wt, np.float32)
+        check(lib().vs_volume_upload(self.h, _ptr(F), _ptr(Wt)), "vs_volume_upload")
+
+    def icp_system(self, level, T21, want_mask=False):
+        T21 = np.ascontiguousarray(T21, np.float32).reshape(16)
+        ATA = np.zeros(36, np.float64)
+        ATb = np.zeros(6, np.float64)
+        n = C.c_longlong()
+        mask = np.zeros(self.level_shape(level), np.uint8) if want_mask else None
+        check(lib().vs_icp_system(self.h, level, _ptr(T21), _ptr(ATA), _ptr(ATb), C.byref(n), _ptr(mask)),
+              "vs_icp_system")
+        return ATA.reshape(6, 6), ATb, n.value, mask
+
+    def set_timing(self, on):
+        check(lib().vs_session_set_timing(self.h, 1 if on else 0), "vs_session_set_timing")
+
+    def stage_ms(self):
+        ms = np.zeros(5, np.float32)
+        check(lib().vs_get_stage_ms(self.h, _ptr(ms)), "vs_get_stage_ms")
+        return ms
+
+    def integrate_counts(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        check(lib().vs_get_integrate_counts(self.h, C.byref(a), C.byref(b)), "vs_get_integrate_counts")
+        return a.value, b.value
+
+    def raycast_counts(self, level):
+        c = np.zeros(3, np.uint64)
+        check(lib().vs_raycast_counts(self.h, level, _ptr(c)), "vs_raycast_counts")
+        return c
+
+
+def session_bytes(desc):
+    return lib().vs_session_bytes(C.byref(desc))
