@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Fused-frame throughput of the batched fusion step on one MI355X.
+
+Workload (the headline metric's configuration): 64 independent scan sessions,
+each a 640x480 uint16 depth stream of its own seeded random-primitive scene and
+orbit, each fusing into its own 512^3 TSDF (FP16 value + u8 weight, cap 128)
+over a 1.5 m cube. One step = one fused frame (bilateral -> pyramid -> ICP
+(3 levels, 10/5/4 iterations) -> brick-culled integration -> raycast of 3
+levels) for every session, launched as one batch. Depth frames are rendered on
+the host beforehand and resident in HBM when the timed region starts.
+
+Prints one JSON line (stdout). Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vcl-volume-scanner_amd"))
+
+import hiprt  # noqa: E402
+import vscan  # noqa: E402
+
+METRIC = "fused frames/s at 640x480/512^3 TSDF; p50/p99 frame latency, 64 sessions"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_sessions(ctx, n_sess, frames, W, H, vol_n):
+    w = vscan.WORKLOADS["fp16_640x480_512"]
+    sessions, streams, bufs = [], [], []
+    for i in range(n_sess):
+        st = vscan.SceneStream(3, seed=1000 + i, width=W, height=H)
+        d = vscan.make_desc(w, K=st.K, n=vol_n, W=W, H=H)
+        s = vscan.Session(ctx, d, T_gc0=st.pose(0))
+        fr = np.stack([st.frame(k) for k in range(frames)])
+        buf = hiprt.DeviceBuffer(fr.nbytes)
+        buf.upload(np.ascontiguousarray(fr))
+        sessions.append(s)
+        streams.append(st)
+        bufs.append(buf)
+        if (i + 1) % 16 == 0:
+            log(f"  sessions ready: {i + 1}/{n_sess}")
+    return sessions, streams, bufs
+
+
+def frame_ptrs(bufs, k, W, H):
+    off = k * W * H * 2
+    return [b.ptr.value + off for b in bufs]
+
+
+def integrate_bytes(sessions, sv_sw):
+    """Algorithmic bytes of one integrate launch over the batch:
+    N_vis*(s_v+s_w) + N_upd*(s_v+s_w) + W*H*4 per session."""
+    tot = 0
+    for s in sessions:
+        vb, nu = s.integrate_counts()
+        tot += (vb * 512 + nu) * sv_sw + s.W * s.H * 4
+    return tot
+
+
+def raycast_bytes_level(sessions, level, sv_sw):
+    """Algorithmic bytes of one raycast launch at `level`:
+    sum_rays (n_near + 8 n_tri + 32 hit)(s_v+s_w) + W_l*H_l*29 per session."""
+    tot = 0
+    for s in sessions:
+        n_near, n_tri, hit = (int(x) for x in s.raycast_counts(level))
+        h, w = s.level_shape(level)
+        tot += (n_near + 8 * n_tri + 32 * hit) * sv_sw + w * h * 29
+    return tot
+
+
+def cpu_baseline(frames):
+    """The reference algorithm's CPU path (the oracle instantiated in FP32 with
+    OpenMP) on one session of the same kind: 640x480, 512^3 volume held in FP32
+    (the CPU path has no FP16 storage)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+
+    w = vscan.WORKLOADS["fp16_640x480_512"]
+    st = vscan.SceneStream(3, seed=1000)
+    d = vscan.make_desc(w, K=st.K)
+    o = orc.OracleSession(d, st.pose(0), fp32=True)
+    cores = orc.oracle().or_num_threads()
+    mm = [st.frame(k) for k in range(frames)]
+    o.fuse(mm[0])  # first frame (no tracking) is warm-up
+    t0 = time.perf_counter()
+    for k in range(1, frames):
+        o.fuse(mm[k])
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": (frames - 1) / dt, "unit": "fused frames/s", "cores": cores, "kind": "port",
+            "sample": f"1 session x {frames - 1} tracked frames of 640x480 after 1 warm-up frame, 512^3 volume "
+                      f"in FP32 (same scene family), OpenMP {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--sessions", type=int, default=64)
+    ap.add_argument("--volume", type=int, default=512)
+    ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build shards sessions across GPUs only by running one process per "
+              "GPU, and multi-GPU runs are not part of this measurement (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    W, H = 640, 480
+    K, Wm = args.steps, args.warmup
+    frames = Wm + K + 2
+    log(f"bench: {args.sessions} sessions, {W}x{H}, {args.volume}^3 FP16+u8, warmup {Wm}, steps {K}")
+    ctx = vscan.Context(0)
+    sessions, streams, bufs = make_sessions(ctx, args.sessions, frames, W, H, args.volume)
+    stream = hiprt.Stream()
+
+    def step(k):
+        ctx.fuse_batch(sessions, frame_ptrs(bufs, k, W, H), is_host=False, stream=stream.h)
+
+    log("warmup")
+    for k in range(Wm):
+        step(k)
+    stream.sync()
+    hiprt.device_sync()
+    ev = [hiprt.Event() for _ in range(K + 1)]
+    log("timed region")
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(K):
+        step(Wm + i)
+        ev[i + 1].record(stream)
+    stream.sync()
+    hiprt.device_sync()
+    wall = time.perf_counter() - t0
+    lat = np.array([ev[i].elapsed_ms(ev[i + 1]) for i in range(K)])
+    gpu_ms = float(lat.sum())
+    n = args.sessions
+    value = n * K / wall
+    statuses = [s.status()[0] for s in sessions]
+    log(f"timed: {wall * 1e3:.1f} ms wall, {gpu_ms:.1f} ms GPU, {value:.1f} fused frames/s, "
+        f"lost sessions: {sum(1 for x in statuses if x != 0)}")
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "fused frames/s", "n_gpus": 1, "steps": K,
+        "warmup": Wm, "ms_per_step": round(wall * 1e3 / K, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} sessions x 640x480 uint16 depth, {args.volume}^3 FP16 TSDF + u8 weight "
+                               f"(cap 128) over 1.5 m, seeded random-primitive scenes and orbits",
+                   "sessions": n, "width": W, "height": H, "volume": args.volume, "tsdf": "f16", "weight": "u8",
+                   "icp_iters": [10, 5, 4]},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "definition": "GPU time of the batched step that fuses the frame (all frames of a step "
+                                     "arrive when it starts)"},
+        "gpu_ms_per_step": round(gpu_ms / K, 3),
+    }
+
+    if not args.no_roofline:
+        # measurement pass: eager launches with events between kernels
+        ctx.set_timing(True)
+        reps = 3
+        acc = np.zeros(8)
+        integ_b = ray_b = 0
+        for r in range(reps):
+            step(Wm + K + (r % 2))
+            acc += ctx.stage_ms()
+        acc /= reps
+        ctx.set_timing(False)
+        sv_sw = 3
+        integ_b = integrate_bytes(sessions, sv_sw)
+        ray_b = raycast_bytes_level(sessions, 0, sv_sw)
+        stage = dict(zip(["reset_bilateral", "pyramid", "icp", "brick_visibility", "integrate", "raycast_l0",
+                          "raycast_l1", "raycast_l2"], [round(float(x), 4) for x in acc]))
+        out["stage_ms"] = stage
+        cand = {"k_integrate": (integ_b, acc[4]), "k_raycast(level 0)": (ray_b, acc[5])}
+        name = max(cand, key=lambda k: cand[k][1])
+        b, ms = cand[name]
+        ach = b / (ms * 1e-3) / 1e9
+        out["roofline"] = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                           "algorithmic_bytes_per_launch": int(b), "avg_launch_ms": round(float(ms), 4)}
+        out["other_kernels"] = {k: {"GB/s": round(v[0] / (v[1] * 1e-3) / 1e9, 1), "ms": round(float(v[1]), 4),
+                                    "bytes": int(v[0])} for k, v in cand.items()}
+        upd = sum(s.integrate_counts()[1] for s in sessions)
+        out["voxel_updates_per_s"] = round(upd / (acc.sum() * 1e-3), 1)
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_frames)
+    for s in sessions:
+        s.close()
+    for b in bufs:
+        b.free()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

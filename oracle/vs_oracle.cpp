@@ -100,6 +100,7 @@ struct Vol {
     int W, H, D;
     f3 origin;
     float vox_inv;
+    float voxel;
     float mu, max_w;
     int BW, BH, BD;
 };
@@ -110,6 +111,7 @@ Vol make_vol(const or_volume *v) {
     o.D = v->dim[2];
     o.origin = {v->origin[0], v->origin[1], v->origin[2]};
     o.vox_inv = 1.0f / v->voxel;
+    o.voxel = v->voxel;
     o.mu = v->mu;
     o.max_w = v->max_w;
     o.BW = (o.W + 7) / 8;
@@ -261,7 +263,8 @@ void vnmap(const float *depth, int W, int H, const float *K, int inplace, R *vtx
 // ---------------------------------------------------------------------------
 // Bilinear vertex taps; FP32 because the result gates the correspondence.
 inline bool interp_vertex(float xf, float yf, int W, int H, const float *m, f3 &out) {
-    if (xf < 0.0f || xf > float(W - 1) || yf < 0.0f || yf > float(H - 1)) return false;
+    // reference bounds test; written so that a NaN coordinate is rejected (Q11)
+    if (!(xf >= 0.0f && xf <= float(W - 1) && yf >= 0.0f && yf <= float(H - 1))) return false;
     int x = (int)xf, y = (int)yf;
     float rx = xf - (float)x, ry = yf - (float)y;
     f3 v1{0, 0, 0}, v2{0, 0, 0}, v3{0, 0, 0}, v4{0, 0, 0};
@@ -290,7 +293,8 @@ inline bool interp_vertex(float xf, float yf, int W, int H, const float *m, f3 &
 // Bilinear normal taps; validity in FP32, the interpolated value in R.
 template <class R>
 inline bool interp_normal(float xf, float yf, int W, int H, const float *m, R n[3]) {
-    if (xf < 0.0f || xf > float(W - 1) || yf < 0.0f || yf > float(H - 1)) return false;
+    // reference bounds test; written so that a NaN coordinate is rejected (Q11)
+    if (!(xf >= 0.0f && xf <= float(W - 1) && yf >= 0.0f && yf <= float(H - 1))) return false;
     int x = (int)xf, y = (int)yf;
     float rx = xf - (float)x, ry = yf - (float)y;
     R t[4][3] = {};
@@ -684,7 +688,7 @@ RayParams ray_params(const Vol &V, const float *K, const float *Tgc, const float
     std::memcpy(P.Tcg, Tcg, sizeof(P.Tcg));
     P.cam = camera_centre(Tgc);
     // cube centre = origin + dim*voxel/2 (z_yooji_object_scanner.cpp:306)
-    const float vox = 1.0f / V.vox_inv;
+    const float vox = V.voxel;
     f3 cen = {V.origin.x + (float)V.W * vox / 2.0f, V.origin.y + (float)V.H * vox / 2.0f,
               V.origin.z + (float)V.D * vox / 2.0f};
     f3 vcc = sub(cen, P.cam);

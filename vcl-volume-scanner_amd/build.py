@@ -26,7 +26,7 @@ HIP_SOURCES = ["vs_img.hip", "vs_icp.hip", "vs_vol.hip", "vs_api.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result", "-Wno-unused-value",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 

@@ -142,11 +142,35 @@ struct Batch {
     std::vector<vs_session *> members;
 };
 
+// Events around the stages of one frame step:
+//   [0,1) track setup + bilateral   [1,2) pyramid    [2,3) ICP + finalize
+//   [3,4) brick clear + mark        [4,5) integrate  [5,6) raycast L0
+//   [6,7) raycast L1                [7,8) raycast L2
+constexpr int kMarks = 9;
+struct Timer {
+    hipEvent_t ev[kMarks] = {};
+    bool used = false;
+    hipError_t init() {
+        for (auto &e : ev)
+            if (!e) {
+                hipError_t r = hipEventCreate(&e);
+                if (r != hipSuccess) return r;
+            }
+        return hipSuccess;
+    }
+    void destroy() {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 struct vs_ctx {
     int device = 0;
     std::map<std::vector<vs_session *>, std::unique_ptr<Batch>> batches;
     std::vector<vs_session *> sessions;
     bool use_graph = true;
+    bool timing = false;
+    Timer timer;
 };
 
 struct vs_session {
@@ -160,8 +184,7 @@ struct vs_session {
     hipStream_t last = nullptr;
     Batch self;
     bool timing = false;
-    hipEvent_t ev[6] = {};
-    float stage_ms[5] = {0, 0, 0, 0, 0};
+    Timer timer;
 };
 
 namespace {
@@ -192,9 +215,9 @@ hipStream_t pick(vs_session *s, vs_stream st) {
 }
 
 // The per-frame launch sequence of a batch (so_Scan_Object order).
-void enqueue_step(const Batch &B, hipStream_t s, vs_session *timed) {
+void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     auto mark = [&](int i) {
-        if (timed) hipEventRecord(timed->ev[i], s);
+        if (timed) (void)hipEventRecord(timed->ev[i], s);
     };
     mark(0);
     launch_track_setup(B.dP, B.n, nullptr, s);
@@ -208,14 +231,19 @@ void enqueue_step(const Batch &B, hipStream_t s, vs_session *timed) {
     mark(3);
     launch_vis_clear(B.dP, B.n, B.bl, s);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, s);
-    launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, s);
     mark(4);
-    for (int l = 0; l < B.levels; ++l) launch_raycast(B.dP, B.dWork[l], B.nWork[l], l, B.tt, B.wt, s);
+    launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, s);
     mark(5);
+    for (int l = 0; l < kMaxLevels; ++l) {
+        if (l < B.levels) launch_raycast(B.dP, B.dWork[l], B.nWork[l], l, B.tt, B.wt, s);
+        mark(6 + l);
+    }
+    if (timed) timed->used = true;
 }
 
-vs_status run_step(vs_ctx *ctx, Batch &B, hipStream_t s, vs_session *timed) {
+vs_status run_step(vs_ctx *ctx, Batch &B, hipStream_t s, Timer *timed) {
     if (timed || !ctx->use_graph) {
+        if (timed) HIP_TRY(timed->init());
         enqueue_step(B, s, timed);
         HIP_TRY(hipGetLastError());
         return VS_OK;
@@ -317,6 +345,7 @@ vs_status vs_ctx_destroy(vs_ctx *ctx) {
     ctx->batches.clear();
     std::vector<vs_session *> ss = ctx->sessions;
     for (vs_session *s : ss) vs_session_destroy(s);
+    ctx->timer.destroy();
     delete ctx;
     return VS_OK;
 }
@@ -460,8 +489,7 @@ vs_status vs_session_destroy(vs_session *s) {
     hipStreamSynchronize(s->stream);
     drop_batches_with(s->ctx, s);
     free_batch_graph(s->self);
-    for (auto &e : s->ev)
-        if (e) hipEventDestroy(e);
+    s->timer.destroy();
     if (s->stream) hipStreamDestroy(s->stream);
     if (s->mem) hipFree(s->mem);
     auto &v = s->ctx->sessions;
@@ -565,11 +593,7 @@ vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs
     hipStream_t h = pick(s, stream);
     vs_status r = copy_depth(s, depth_mm, is_host, h);
     if (r != VS_OK) return r;
-    if (s->timing) {
-        for (auto &e : s->ev)
-            if (!e) HIP_TRY(hipEventCreate(&e));
-    }
-    return run_step(s->ctx, s->self, h, s->timing ? s : nullptr);
+    return run_step(s->ctx, s->self, h, s->timing ? &s->timer : nullptr);
 }
 
 vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t *const *depths, int n, int is_host,
@@ -640,7 +664,7 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
         if (r != VS_OK) return r;
         key[i]->last = h;
     }
-    return run_step(ctx, *B, h, nullptr);
+    return run_step(ctx, *B, h, ctx->timing ? &ctx->timer : nullptr);
 }
 
 vs_status vs_get_frame_status(vs_session *s, int *last_status, int *frames_fused, int *icp_iters) {
@@ -819,10 +843,30 @@ vs_status vs_session_set_timing(vs_session *s, int enable) {
 
 vs_status vs_get_stage_ms(vs_session *s, float ms[5]) {
     if (!s || !ms) return fail(VS_EINVAL, "null argument");
-    if (!s->timing || !s->ev[0]) return fail(VS_EINVAL, "timing not enabled");
+    if (!s->timing || !s->timer.used) return fail(VS_EINVAL, "timing not enabled");
     vs_status r = sync_last(s);
     if (r != VS_OK) return r;
-    for (int i = 0; i < 5; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], s->ev[i], s->ev[i + 1]));
+    float k[kMarks - 1];
+    for (int i = 0; i < kMarks - 1; ++i) HIP_TRY(hipEventElapsedTime(&k[i], s->timer.ev[i], s->timer.ev[i + 1]));
+    ms[0] = k[0];                 // bilateral (with the tracker reset)
+    ms[1] = k[1];                 // pyramid
+    ms[2] = k[2];                 // track
+    ms[3] = k[3] + k[4];          // integrate (visibility + update)
+    ms[4] = k[5] + k[6] + k[7];   // raycast, all levels
+    return VS_OK;
+}
+
+vs_status vs_ctx_set_timing(vs_ctx *ctx, int enable) {
+    if (!ctx) return fail(VS_EINVAL, "null context");
+    ctx->timing = enable != 0;
+    return VS_OK;
+}
+
+vs_status vs_ctx_stage_ms(vs_ctx *ctx, float ms[8]) {
+    if (!ctx || !ms) return fail(VS_EINVAL, "null argument");
+    if (!ctx->timing || !ctx->timer.used) return fail(VS_EINVAL, "timing not enabled");
+    HIP_TRY(hipEventSynchronize(ctx->timer.ev[kMarks - 1]));
+    for (int i = 0; i < kMarks - 1; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], ctx->timer.ev[i], ctx->timer.ev[i + 1]));
     return VS_OK;
 }
 

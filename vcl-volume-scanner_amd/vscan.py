@@ -81,7 +81,7 @@ def lib():
                      "vs_fuse_frame", "vs_fuse_batch", "vs_get_frame_status", "vs_get_pose", "vs_set_pose",
                      "vs_get_filtered", "vs_set_filtered", "vs_get_maps", "vs_set_maps", "vs_volume_download",
                      "vs_volume_upload", "vs_icp_system", "vs_session_set_timing", "vs_get_stage_ms",
-                     "vs_get_integrate_counts", "vs_raycast_counts"]:
+                     "vs_get_integrate_counts", "vs_raycast_counts", "vs_ctx_set_timing", "vs_ctx_stage_ms"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
@@ -109,6 +109,8 @@ def lib():
         L.vs_get_stage_ms.argtypes = [P, P]
         L.vs_get_integrate_counts.argtypes = [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.vs_raycast_counts.argtypes = [P, C.c_int, P]
+        L.vs_ctx_set_timing.argtypes = [P, C.c_int]
+        L.vs_ctx_stage_ms.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -254,6 +256,14 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_timing(self, on):
+        check(lib().vs_ctx_set_timing(self.h, 1 if on else 0), "vs_ctx_set_timing")
+
+    def stage_ms(self):
+        ms = np.zeros(8, np.float32)
+        check(lib().vs_ctx_stage_ms(self.h, _ptr(ms)), "vs_ctx_stage_ms")
+        return ms
 
     def fuse_batch(self, sessions, depths, is_host=True, stream=None):
         n = len(sessions)
