@@ -51,6 +51,35 @@ def make_sessions(ctx, n_sess, frames, W, H, vol_n):
     return sessions, streams, bufs
 
 
+def plain_serial(ctx, streams, bufs, W, H, vol_n, warm, steps, stream):
+    """The plain form: FP32 value + FP32 weight volumes, each session fused on
+    its own (vs_fuse_frame, one graph per session), one after the other, on
+    the same resident frames. Returns fused frames/s."""
+    w = vscan.WORKLOADS["fp16_640x480_512"]
+    sess = []
+    for st in streams:
+        d = vscan.make_desc(w, tsdf=vscan.VS_TSDF_F32, weight=vscan.VS_WEIGHT_F32, K=st.K, n=vol_n, W=W, H=H)
+        d.max_weight = 100.0
+        sess.append(vscan.Session(ctx, d, T_gc0=st.pose(0)))
+    L = vscan.lib()
+
+    def run(k):
+        for s, b in zip(sess, bufs):
+            vscan.check(L.vs_fuse_frame(s.h, vscan.P(b.ptr.value + k * W * H * 2), 0, stream.h), "vs_fuse_frame")
+
+    for k in range(warm):
+        run(k)
+    stream.sync()
+    t0 = time.perf_counter()
+    for k in range(warm, warm + steps):
+        run(k)
+    stream.sync()
+    dt = time.perf_counter() - t0
+    for s in sess:
+        s.close()
+    return len(streams) * steps / dt
+
+
 def frame_ptrs(bufs, k, W, H):
     off = k * W * H * 2
     return [b.ptr.value + off for b in bufs]
@@ -111,6 +140,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build shards sessions across GPUs only by running one process per "
@@ -192,6 +222,16 @@ def main():
                                     "bytes": int(v[0])} for k, v in cand.items()}
         upd = sum(s.integrate_counts()[1] for s in sessions)
         out["voxel_updates_per_s"] = round(upd / (acc.sum() * 1e-3), 1)
+    if not args.no_plain:
+        log("plain FP32 one-session-at-a-time pipeline")
+        for s in sessions:  # free the FP16 volumes first
+            s.close()
+        sessions = []
+        pf = plain_serial(ctx, streams, bufs, W, H, args.volume, 2, min(K, 5), stream)
+        out["plain_serial_fp32"] = {"value": round(pf, 2), "unit": "fused frames/s",
+                                    "definition": "same sessions and frames, FP32 value + FP32 weight volumes, "
+                                                  "each session fused alone (vs_fuse_frame) one after another"}
+        out["speedup_vs_plain_serial"] = round(value / pf, 3)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_frames)

@@ -1,0 +1,115 @@
+"""Batched execution and the precision form.
+
+* A session gives bit-identical results alone (vs_fuse_frame) and inside a
+  batch with other sessions of different frame and volume sizes
+  (vs_fuse_batch): per-session reductions are summed in a fixed tile order and
+  every brick and pixel is computed independently of its neighbours in the
+  batch.
+* The FP16-value / u8-weight volume tracks the FP64 oracle within
+  0.5 mm / 0.05 deg and keeps the surface within 1 mm.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+import vscan
+
+pytestmark = pytest.mark.gpu
+
+
+def _sessions(ctx, tsdf, weight, specs):
+    out = []
+    for cfg, seed, W, H, n in specs:
+        st = vscan.SceneStream(cfg, seed=seed, width=W, height=H)
+        w = dict(vscan.WORKLOADS["plain_320x240_128"])
+        if tsdf == vscan.VS_TSDF_F16:
+            w["max_w"] = 128.0
+        d = vscan.make_desc(w, tsdf=tsdf, weight=weight, n=n, W=W, H=H, K=st.K)
+        out.append((st, d))
+    return out
+
+
+@pytest.mark.parametrize("types", [(vscan.VS_TSDF_F32, vscan.VS_WEIGHT_F32), (vscan.VS_TSDF_F16, vscan.VS_WEIGHT_U8)])
+def test_batch_equals_alone(ctx, types):
+    print(f"case: batch vs alone, types {types}", flush=True)
+    specs = [(4, 1, 320, 240, 128), (3, 1001, 640, 480, 64), (3, 1002, 320, 240, 96)]
+    made = _sessions(ctx, types[0], types[1], specs)
+    alone = [vscan.Session(ctx, d, T_gc0=st.pose(0)) for st, d in made]
+    mixed = [vscan.Session(ctx, d, T_gc0=st.pose(0)) for st, d in made]
+    try:
+        for fr in range(6):
+            frames = [st.frame(fr) for st, _ in made]
+            for s, f in zip(alone, frames):
+                s.fuse(f)
+            ctx.fuse_batch(mixed, frames)
+            for a, m in zip(alone, mixed):
+                assert a.status() == m.status()
+                assert np.array_equal(a.pose(), m.pose()), f"frame {fr}: pose differs"
+        for a, m in zip(alone, mixed):
+            Fa, Wa, Ba = a.volume()
+            Fm, Wm, Bm = m.volume()
+            assert np.array_equal(Fa, Fm) and np.array_equal(Wa, Wm) and np.array_equal(Ba, Bm)
+            for l in range(3):
+                assert all(np.array_equal(x, y) for x, y in zip(a.maps(l), m.maps(l)))
+    finally:
+        for s in alone + mixed:
+            s.close()
+
+
+def test_fp16_form_tracks_fp64_oracle(ctx):
+    print("case: FP16 + u8 volume vs FP64 oracle, 320x240 / 128^3, 20 frames", flush=True)
+    st = vscan.SceneStream(4, seed=1)
+    w = dict(vscan.WORKLOADS["plain_320x240_128"])
+    w["max_w"] = 128.0
+    d = vscan.make_desc(w, tsdf=vscan.VS_TSDF_F16, weight=vscan.VS_WEIGHT_U8, K=st.K)
+    d64 = vscan.make_desc(w, K=st.K)
+    s = vscan.Session(ctx, d, T_gc0=st.pose(0))
+    o = orc.OracleSession(d64, st.pose(0))
+    worst = (0.0, 0.0)
+    try:
+        for fr in range(20):
+            mm = st.frame(fr)
+            s.fuse(mm)
+            o.fuse(mm)
+            assert s.status()[0] == 0
+            dt, da = orc.pose_error(s.pose(), o.pose())
+            worst = (max(worst[0], dt), max(worst[1], da))
+        print(f"FP16 form worst pose gap: {worst[0] * 1e3:.4f} mm, {worst[1]:.5f} deg")
+        assert worst[0] <= 0.5e-3 and worst[1] <= 0.05
+        gF, gW, _ = s.volume()
+        oF, oW = o.volume()
+        seen = (gW > 0) & (oW > 0)
+        err = np.abs(gF - oF)[seen]
+        print(f"FP16 TSDF error vs FP64: median {np.median(err):.2e}, p99 {np.percentile(err, 99):.2e}")
+        _, gv, _, _ = s.maps(0)
+        _, ov, _ = o.maps(0)
+        both = (gv[..., 0] != -100) & (ov[..., 0] != -100)
+        surf = np.linalg.norm(gv - ov, axis=-1)[both]
+        print(f"FP16 surface gap: p50 {np.median(surf) * 1e3:.4f} mm, p99 {np.percentile(surf, 99) * 1e3:.4f} mm")
+        assert np.percentile(surf, 99) <= 1e-3
+    finally:
+        s.close()
+        o.close()
+
+
+def test_failure_keeps_pose_and_skips_fusion(ctx):
+    """An all-zero frame after the first one: the solve fails (J^T J = 0), the
+    pose is kept and the volume is untouched (so_Scan_Object's failure path)."""
+    print("case: tracking failure on an empty frame", flush=True)
+    st = vscan.SceneStream(4, seed=2)
+    d = vscan.make_desc(vscan.WORKLOADS["plain_320x240_128"], K=st.K)
+    s = vscan.Session(ctx, d, T_gc0=st.pose(0))
+    try:
+        s.fuse(st.frame(0))
+        T1 = s.pose()
+        F1, W1, _ = s.volume()
+        s.fuse(np.zeros((d.height, d.width), np.uint16))
+        status, frames, solves = s.status()
+        assert status == vscan.VS_TRACK_LOST and frames == 1 and solves == 1
+        assert np.array_equal(s.pose(), T1)
+        F2, W2, _ = s.volume()
+        assert np.array_equal(F1, F2) and np.array_equal(W1, W2)
+        s.fuse(st.frame(1))  # recovers on the next real frame
+        assert s.status()[0] == 0
+    finally:
+        s.close()

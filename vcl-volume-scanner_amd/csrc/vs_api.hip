@@ -659,11 +659,26 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
     } else {
         B = it->second.get();
     }
-    for (int i = 0; i < n; ++i) {
-        vs_status r = copy_depth(key[i], depths[i], is_host, h);
-        if (r != VS_OK) return r;
-        key[i]->last = h;
+    if (is_host) {
+        for (int i = 0; i < n; ++i) {
+            vs_status r = copy_depth(key[i], depths[i], 1, h);
+            if (r != VS_OK) return r;
+        }
+    } else {
+        // device-resident frames: one gather launch per 64 sessions
+        for (int first = 0; first < n; first += kDepthChunk) {
+            DepthChunk ch{};
+            ch.first = first;
+            ch.n = std::min(kDepthChunk, n - first);
+            for (int i = 0; i < ch.n; ++i) {
+                if (!depths[first + i]) return fail(VS_EINVAL, "null depth for session %d", first + i);
+                ch.src[i] = depths[first + i];
+            }
+            launch_gather_depth(B->dP, ch, h);
+        }
+        HIP_TRY(hipGetLastError());
     }
+    for (int i = 0; i < n; ++i) key[i]->last = h;
     return run_step(ctx, *B, h, ctx->timing ? &ctx->timer : nullptr);
 }
 

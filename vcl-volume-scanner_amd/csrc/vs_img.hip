@@ -136,6 +136,32 @@ __global__ __launch_bounds__(kTileThreads) void k_vnmap(const SessParams *__rest
     nrm[3 * t + 0] = o[3]; nrm[3 * t + 1] = o[4]; nrm[3 * t + 2] = o[5];
 }
 
+// Copies the caller's device-resident frames into the sessions' raw-frame
+// buffers: one launch for a whole batch (blockIdx.y = session of the chunk),
+// 16-byte vector loads/stores where the frame size allows.
+__global__ __launch_bounds__(256) void k_gather_depth(const SessParams *__restrict__ SP, DepthChunk ch) {
+    const int s = blockIdx.y;
+    if (s >= ch.n) return;
+    const SessParams &P = SP[ch.first + s];
+    const size_t npx = (size_t)P.lev[0].W * P.lev[0].H;
+    const uint16_t *src = ch.src[s];
+    uint16_t *dst = const_cast<uint16_t *>(P.raw);
+    const bool vec = ((npx & 7) == 0) && ((((uintptr_t)src) & 15) == 0);
+    if (vec) {
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+        uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx / 8; i += (size_t)gridDim.x * blockDim.x)
+            d4[i] = s4[i];
+    } else {
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += (size_t)gridDim.x * blockDim.x)
+            dst[i] = src[i];
+    }
+}
+
+void launch_gather_depth(const SessParams *P, const DepthChunk &ch, hipStream_t s) {
+    if (ch.n > 0) hipLaunchKernelGGL(k_gather_depth, dim3(64, ch.n), dim3(256), 0, s, P, ch);
+}
+
 void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_bilateral, dim3(n), dim3(kTileThreads), 0, s, P, work);
 }

@@ -8,6 +8,12 @@
 namespace vs {
 
 // image stages (vs_img.hip)
+constexpr int kDepthChunk = 64;
+struct DepthChunk {  // passed by value: device frame pointers of sessions [first, first+n)
+    const uint16_t *src[kDepthChunk];
+    int first, n;
+};
+void launch_gather_depth(const SessParams *P, const DepthChunk &ch, hipStream_t s);
 void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s);
 void launch_pyr_down(const SessParams *P, const WorkItem *work, int n, int level, hipStream_t s);
 void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int level, float *vtx, float *nrm, hipStream_t s);

@@ -140,11 +140,22 @@ struct Vol {
     const SessParams &P;
     const TV *F;
     const TW *Wt;
+    // occupancy bit of the last brick looked up (a ray's consecutive taps
+    // mostly stay in one brick); bocc is read-only during the launch
+    mutable int cb = -1;
+    mutable bool cocc = false;
+    __device__ __forceinline__ bool occupied(int b) const {
+        if (b != cb) {
+            cb = b;
+            cocc = (P.bocc[b >> 5] >> (b & 31)) & 1u;
+        }
+        return cocc;
+    }
     __device__ __forceinline__ bool tap(int x, int y, int z, float &out) const {
         out = 1.0f;
         if (x < 0 || x >= P.vdim[0] || y < 0 || y >= P.vdim[1] || z < 0 || z >= P.vdim[2]) return false;
         const int b = brick_of(P, x, y, z);
-        if (!(P.bocc[b >> 5] & (1u << (b & 31)))) return false;
+        if (!occupied(b)) return false;
         const size_t idx = (size_t)b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
         if (!(ld_w(Wt, idx) > 0.0f)) return false;
         out = ld_val(F, idx);
