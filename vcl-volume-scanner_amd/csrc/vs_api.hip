@@ -226,7 +226,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     for (int l = 1; l < B.levels; ++l) launch_pyr_down(B.dP, B.dWork[l], B.nWork[l], l, s);
     mark(2);
     for (int l = B.levels - 1; l >= 0; --l)
-        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], l, it, s);
+        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], B.n, l, it, s);
     launch_track_finalize(B.dP, B.n, s);
     mark(3);
     launch_vis_clear(B.dP, B.n, B.bl, s);
@@ -559,7 +559,7 @@ vs_status vs_track(vs_session *s, const double *T_01_init, vs_stream stream) {
     const Batch &B = s->self;
     launch_track_setup_forced(B.dP, 1, T_01_init, h);
     for (int l = B.levels - 1; l >= 0; --l)
-        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], l, it, h);
+        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], B.n, l, it, h);
     launch_track_finalize(B.dP, 1, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;

@@ -50,7 +50,6 @@ struct alignas(16) SessState {
     int last_status;   // status of the last frame
     int solves;        // Gauss-Newton solves of the last frame
     int force_single;  // T_init given: only level 0
-    unsigned icp_counter;       // last-block counter of the ICP reduction
     unsigned vis_count;         // visible bricks of the last integrate
     unsigned long long upd_count;  // voxels updated by the last integrate
     double last_ATA[36];        // normal equations of the last solve (diagnostic)

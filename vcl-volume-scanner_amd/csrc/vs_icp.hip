@@ -217,18 +217,24 @@ __device__ __forceinline__ void block_terms(float v[kIcpVals], double *red64) {
     __syncthreads();
 }
 
-// One Gauss-Newton iteration at `level` for every session of the work list.
+__device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *st, int level, int iter) {
+    return !st->do_track || st->icp_failed || st->conv_level == level || iter >= P.iters[level] ||
+           (st->force_single && level != 0);
+}
+
+// One Gauss-Newton iteration at `level`, first half: the 29 sums of every
+// 32x8 tile of every session of the work list, one FP64 partial per tile.
+// No cross-workgroup synchronisation here: on a multi-XCD device an
+// agent-scope release per workgroup would write back L2 each time, so the
+// reduction and the solve are the next launch (k_icp_solve).
 __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__restrict__ SP,
                                                            const WorkItem *__restrict__ work, int level, int iter) {
     __shared__ float T21s[16];
     __shared__ double red64[kIcpVals];
-    __shared__ int is_last;
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
-    SessState *st = P.st;
-    if (!st->do_track || st->icp_failed || st->conv_level == level || iter >= P.iters[level] ||
-        (st->force_single && level != 0))
-        return;
+    const SessState *st = P.st;
+    if (icp_skip(P, st, level, iter)) return;
     if (threadIdx.x < 16) T21s[threadIdx.x] = st->T_21_f[threadIdx.x];
     __syncthreads();
     const LevelDesc &L = P.lev[level];
@@ -240,22 +246,28 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     float v[kIcpVals];
     pixel_terms(ok, A, b, v);
     block_terms(v, red64);
-    const int ntiles = L.tiles_x * L.tiles_y;
     if (threadIdx.x < kIcpVals) P.icp_part[(size_t)wi.tile * kIcpVals + threadIdx.x] = red64[threadIdx.x];
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(&st->icp_counter, 1u) == (unsigned)(ntiles - 1));
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    // Last workgroup of this session: FP64 sum of all partials in tile order.
+}
+
+// Second half: one workgroup per session sums the tile partials in FP64 in
+// tile order (so the result does not depend on the batch or the schedule),
+// solves the 6x6 system and applies the SE(3) update.
+__global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__restrict__ SP, int nsess, int level,
+                                                            int iter) {
+    const int s = blockIdx.x;
+    if (s >= nsess) return;
+    const SessParams &P = SP[s];
+    SessState *st = P.st;
+    if (level >= P.levels || icp_skip(P, st, level, iter)) return;
+    const LevelDesc &L = P.lev[level];
+    const int ntiles = L.tiles_x * L.tiles_y;
     double acc[kIcpVals];
 #pragma unroll
     for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0;
     for (int t = threadIdx.x; t < ntiles; t += kTileThreads) {
         const double *pp = P.icp_part + (size_t)t * kIcpVals;
 #pragma unroll
-        for (int k = 0; k < kIcpVals; ++k) acc[k] += __builtin_nontemporal_load(pp + k);
+        for (int k = 0; k < kIcpVals; ++k) acc[k] += pp[k];
     }
     __shared__ double wred[kTileThreads / 64][kIcpVals];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -282,7 +294,6 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     for (int i = 0; i < 6; ++i) st->last_ATb[i] = ATb[i];
     st->last_valid = cnt;
     st->solves += 1;
-    st->icp_counter = 0u;
     if (P.icp_min_valid > 0 && cnt < (double)P.icp_min_valid) {
         st->icp_failed = 7;
         return;
@@ -313,7 +324,6 @@ __global__ void k_track_setup(const SessParams *__restrict__ SP, int nsess, Mat1
     st->icp_failed = 0;
     st->conv_level = -1;
     st->solves = 0;
-    st->icp_counter = 0u;
     st->force_single = has_init;
     for (int k = 0; k < 16; ++k) st->T_12[k] = has_init ? Tinit.m[k] : ((k % 5) == 0 ? 1.0 : 0.0);
     publish_T21(st);
@@ -426,8 +436,11 @@ void launch_track_setup_forced(const SessParams *P, int nsess, const double *T_i
         for (int k = 0; k < 16; ++k) m.m[k] = T_init[k];
     hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 1);
 }
-void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int level, int iter, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_icp_iter, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
+void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess, int level, int iter,
+                     hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_icp_iter, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
+    hipLaunchKernelGGL(k_icp_solve, dim3(nsess), dim3(kTileThreads), 0, s, P, nsess, level, iter);
 }
 void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s) {
     hipLaunchKernelGGL(k_track_finalize, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess);

@@ -20,7 +20,8 @@ void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int level, f
 
 // tracker (vs_icp.hip); nsess = sessions in the batch
 void launch_track_setup(const SessParams *P, int nsess, const double *T_init, hipStream_t s);
-void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int level, int iter, hipStream_t s);
+void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess, int level, int iter,
+                     hipStream_t s);
 void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s);
 // one system, no solve: writes 29 doubles (+ optional mask) for diagnostics
 void launch_icp_system(const SessParams *P, const WorkItem *work, int n, int level, const float *T21,
