@@ -53,11 +53,12 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, icp_part, diag_part, diag_mask, blist, bcount, params, work[kMaxLevels],
+    size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, blist, bcount, params, work[kMaxLevels],
         stats;
     size_t total;
     size_t nvox;
-    uint32_t nbricks, nwords;
+    uint32_t nbricks, nwords, nsuper, nswords;
+    int sdim[3];
     int tiles[kMaxLevels];
 };
 
@@ -75,6 +76,9 @@ Layout make_layout(const vs_session_desc &d) {
     L.nvox = (size_t)d.vol_dim[0] * d.vol_dim[1] * d.vol_dim[2];
     L.nbricks = (uint32_t)((d.vol_dim[0] / 8) * (d.vol_dim[1] / 8) * (d.vol_dim[2] / 8));
     L.nwords = (L.nbricks + 31) / 32;
+    for (int k = 0; k < 3; ++k) L.sdim[k] = (d.vol_dim[k] / 8 + 3) / 4;
+    L.nsuper = (uint32_t)(L.sdim[0] * L.sdim[1] * L.sdim[2]);
+    L.nswords = (L.nsuper + 31) / 32;
     const size_t sv = d.tsdf_type == VS_TSDF_F16 ? 2 : 4, sw = d.weight_type == VS_WEIGHT_U8 ? 1 : 4;
     L.state = take(sizeof(SessState));
     L.params = take(sizeof(SessParams));
@@ -93,6 +97,7 @@ Layout make_layout(const vs_session_desc &d) {
     L.weight = take(L.nvox * sw);
     L.bvis = take((size_t)L.nwords * 4);
     L.bocc = take((size_t)L.nwords * 4);
+    L.socc = take((size_t)L.nswords * 4);
     L.icp_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_mask = take((size_t)W * H);
@@ -432,6 +437,9 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.weight = m + L.weight;
     P.bvis = (uint32_t *)(m + L.bvis);
     P.bocc = (uint32_t *)(m + L.bocc);
+    P.socc = (uint32_t *)(m + L.socc);
+    for (int k = 0; k < 3; ++k) P.sdim[k] = L.sdim[k];
+    P.socc_words = (int)L.nswords;
     P.icp_part = (double *)(m + L.icp_part);
     P.st = (SessState *)(m + L.state);
     P.tsdf_type = d.tsdf_type;

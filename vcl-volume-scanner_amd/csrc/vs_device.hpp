@@ -22,6 +22,8 @@ constexpr int kTileW = 32;          // image tile of one 256-thread workgroup: 3
 constexpr int kTileH = 8;
 constexpr int kTileThreads = kTileW * kTileH;
 constexpr int kIcpVals = 29;        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
+constexpr int kSuper = 32;          // superbrick edge (voxels) = 4 bricks
+constexpr int kSoccLdsWords = 1024; // superbrick bits a raycast workgroup keeps in LDS (up to 1024^3 voxels)
 
 struct LevelDesc {
     int W, H;
@@ -87,6 +89,9 @@ struct alignas(16) SessParams {
     void *weight;            // brick-major weights
     uint32_t *bvis;          // visible-brick bits of the current frame
     uint32_t *bocc;          // bricks holding any weight > 0 (monotone)
+    uint32_t *socc;          // superbricks (4^3 bricks = 32^3 voxels) holding an occupied brick
+    int sdim[3];             // superbricks per axis
+    int socc_words;          // 32-bit words of socc
     double *icp_part;        // ICP block partials (kIcpVals per tile of level 0)
     SessState *st;
     int tsdf_type, weight_type;
@@ -150,6 +155,10 @@ __device__ __forceinline__ size_t vox_index(const SessParams &P, int x, int y, i
 }
 __device__ __forceinline__ int brick_of(const SessParams &P, int x, int y, int z) {
     return ((z >> 3) * P.bdim[1] + (y >> 3)) * P.bdim[0] + (x >> 3);
+}
+// superbrick of brick (bx, by, bz)
+__device__ __forceinline__ int super_of_brick(const SessParams &P, int bx, int by, int bz) {
+    return ((bz >> 2) * P.sdim[1] + (by >> 2)) * P.sdim[0] + (bx >> 2);
 }
 
 // Storage conversions for the two volume forms.

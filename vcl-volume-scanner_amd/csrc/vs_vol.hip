@@ -36,12 +36,32 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
 
 // Marks the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
 // pixel's ray and appends each newly marked brick once to the batch's list.
+// Neighbouring pixels hit the same few bricks, so a workgroup first
+// deduplicates its up to 6 x 256 samples in an LDS hash set; only the first
+// insertion of a brick touches the device-wide bitmap (device-scope atomics are
+// performed past the per-XCD L2 and are the expensive part).
+constexpr int kVisSet = 2048;  // > 6 samples x 256 pixels, so probing always ends
+__device__ __forceinline__ void mark_brick(const SessParams &P, int sess, uint32_t b, BrickList bl) {
+    const uint32_t bit = 1u << (b & 31u);
+    uint32_t *word = P.bvis + (b >> 5);
+    if (__atomic_load_n(word, __ATOMIC_RELAXED) & bit) return;
+    const uint32_t old = atomicOr(word, bit);
+    if (!(old & bit)) {
+        const unsigned idx = atomicAdd(bl.count, 1u);
+        if (idx < bl.capacity) bl.items[idx] = make_int2(sess, (int)b);
+        atomicAdd(&P.st->vis_count, 1u);
+    }
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__restrict__ SP,
                                                            const WorkItem *__restrict__ work, BrickList bl) {
+    __shared__ int set[kVisSet];
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
     if (!st->fuse_ok) return;
+    for (int i = threadIdx.x; i < kVisSet; i += kTileThreads) set[i] = -1;
+    __syncthreads();
     const LevelDesc &L = P.lev[0];
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
@@ -62,14 +82,15 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
         const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)(tv.z + 0.5f) / kBrick;
         if (bx < 0 || bx >= P.bdim[0] || by < 0 || by >= P.bdim[1] || bz < 0 || bz >= P.bdim[2]) break;
         const uint32_t b = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;
-        const uint32_t bit = 1u << (b & 31u);
-        uint32_t *word = P.bvis + (b >> 5);
-        if (__atomic_load_n(word, __ATOMIC_RELAXED) & bit) continue;
-        const uint32_t old = atomicOr(word, bit);
-        if (!(old & bit)) {
-            const unsigned idx = atomicAdd(bl.count, 1u);
-            if (idx < bl.capacity) bl.items[idx] = make_int2(wi.sess, (int)b);
-            atomicAdd(&P.st->vis_count, 1u);
+        uint32_t h = (b * 2654435761u) >> 21;  // 11-bit multiplicative hash
+        while (true) {
+            const int old = atomicCAS(&set[h], -1, (int)b);
+            if (old == -1) {
+                mark_brick(P, wi.sess, b, bl);
+                break;
+            }
+            if (old == (int)b) break;
+            h = (h + 1) & (kVisSet - 1);
         }
     }
 }
@@ -123,6 +144,8 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
         const int c1 = __syncthreads_count(nupd >= 1), c2 = __syncthreads_count(nupd >= 2);
         if (t == 0 && c1 > 0) {
             atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
+            const int sb = super_of_brick(P, bx, by, bz);
+            atomicOr(P.socc + (sb >> 5), 1u << (sb & 31));
             atomicAdd(&P.st->upd_count, (unsigned long long)(c1 + c2));
         }
     }
@@ -140,22 +163,30 @@ struct Vol {
     const SessParams &P;
     const TV *F;
     const TW *Wt;
-    // occupancy bit of the last brick looked up (a ray's consecutive taps
-    // mostly stay in one brick); bocc is read-only during the launch
-    mutable int cb = -1;
-    mutable bool cocc = false;
-    __device__ __forceinline__ bool occupied(int b) const {
+    const uint32_t *sbits;  // superbrick bits (LDS copy when it fits)
+    // occupancy of the last brick and superbrick looked up: a ray's
+    // consecutive taps mostly stay in one brick, and empty space is crossed a
+    // superbrick at a time; both bitmaps are read-only during the launch
+    mutable int cb = -1, csb = -1;
+    mutable bool cocc = false, csocc = false;
+    __device__ __forceinline__ bool occupied(int x, int y, int z) const {
+        const int b = brick_of(P, x, y, z);
         if (b != cb) {
             cb = b;
-            cocc = (P.bocc[b >> 5] >> (b & 31)) & 1u;
+            const int sb = super_of_brick(P, x >> 3, y >> 3, z >> 3);
+            if (sb != csb) {
+                csb = sb;
+                csocc = (sbits[sb >> 5] >> (sb & 31)) & 1u;
+            }
+            cocc = csocc && ((P.bocc[b >> 5] >> (b & 31)) & 1u);
         }
         return cocc;
     }
     __device__ __forceinline__ bool tap(int x, int y, int z, float &out) const {
         out = 1.0f;
         if (x < 0 || x >= P.vdim[0] || y < 0 || y >= P.vdim[1] || z < 0 || z >= P.vdim[2]) return false;
-        const int b = brick_of(P, x, y, z);
-        if (!occupied(b)) return false;
+        if (!occupied(x, y, z)) return false;
+        const int b = cb;
         const size_t idx = (size_t)b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
         if (!(ld_w(Wt, idx) > 0.0f)) return false;
         out = ld_val(F, idx);
@@ -316,10 +347,16 @@ template <class TV, class TW, bool STATS>
 __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__restrict__ SP,
                                                           const WorkItem *__restrict__ work, int level,
                                                           unsigned long long *stats) {
+    __shared__ uint32_t s_socc[kSoccLdsWords];
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
     if (!st->fuse_ok) return;
+    const bool lds_bits = P.socc_words <= kSoccLdsWords;
+    if (lds_bits) {
+        for (int i = threadIdx.x; i < P.socc_words; i += kTileThreads) s_socc[i] = P.socc[i];
+        __syncthreads();
+    }
     const LevelDesc &L = P.lev[level];
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
@@ -335,7 +372,7 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
     C.step_scale = P.step_scale;
     C.total_max = P.total_max;
     C.max_iter = P.max_iter;
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight};
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, lds_bits ? s_socc : P.socc};
     unsigned n_near = 0, n_tri = 0;
     f3 p, n;
     const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri) &&
@@ -378,7 +415,13 @@ __global__ void k_volume_init(const SessParams *__restrict__ SP, size_t nvox, ui
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (size_t)gridDim.x * blockDim.x) {
         P.bvis[w] = 0u;
         P.bocc[w] = 0u;
+        if (w < (size_t)P.socc_words) P.socc[w] = 0u;
     }
+}
+
+__global__ void k_socc_clear(const SessParams *__restrict__ SP) {
+    const SessParams &P = SP[0];
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < P.socc_words; w += gridDim.x * blockDim.x) P.socc[w] = 0u;
 }
 
 // brick-major -> linear x-fastest, converted to float
@@ -413,8 +456,14 @@ __global__ __launch_bounds__(kTileThreads) void k_occupancy(const SessParams *__
         const TW *Wt = (const TW *)P.weight + (size_t)b * kBrickVox;
         const int any = __syncthreads_or((ld_w(Wt, threadIdx.x) > 0.0f) || (ld_w(Wt, threadIdx.x + 256) > 0.0f));
         if (threadIdx.x == 0) {
-            if (any) atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
-            else atomicAnd(P.bocc + (b >> 5), ~(1u << (b & 31)));
+            if (any) {
+                atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
+                const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
+                const int sb = super_of_brick(P, bx, by, bz);
+                atomicOr(P.socc + (sb >> 5), 1u << (sb & 31));
+            } else {
+                atomicAnd(P.bocc + (b >> 5), ~(1u << (b & 31)));
+            }
         }
     }
 }
@@ -491,6 +540,7 @@ void launch_volume_from_linear(const SessParams *P, size_t nvox, int tt, int wt,
 void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int wt, hipStream_t s) {
     int g = (int)(nbricks > 16384 ? 16384 : nbricks);
     if (g <= 0) return;
+    hipLaunchKernelGGL(k_socc_clear, dim3(16), dim3(256), 0, s, P);
     if (wt == 0) hipLaunchKernelGGL((k_occupancy<float>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
     else hipLaunchKernelGGL((k_occupancy<uint8_t>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
 }
