@@ -130,6 +130,51 @@ def cpu_baseline(frames):
                       f"in FP32 (same scene family), OpenMP {cores} threads"}
 
 
+def serve(ctx, streams, bufs, frames, W, H, seconds, with_plain):
+    """Serving form: Poisson session arrivals (2/s after 16 at start),
+    lifetimes 150-900 frames at 30 fps +-5 ms, volumes drawn from
+    {128,256,384,512}^3 FP16 + u8, admitted under a 200 GB budget into a
+    128-slot table. Frames stream from the resident pools (forward/backward
+    walk over each scene's frames). The same trace is then run through the
+    plain pipeline (FP32 volumes, one vs_fuse_frame per frame)."""
+    import serving
+
+    w = vscan.WORKLOADS["fp16_640x480_512"]
+    K = streams[0].K
+    reqs = serving.make_requests(7, seconds, len(bufs))
+    ptr = lambda scene, k: bufs[scene].ptr.value + k * W * H * 2  # noqa: E731
+    pose0 = lambda scene: streams[scene].pose(0)  # noqa: E731
+
+    def desc16(n):
+        return vscan.make_desc(w, K=K, n=n, W=W, H=H)
+
+    def desc32(n):
+        d = vscan.make_desc(w, tsdf=vscan.VS_TSDF_F32, weight=vscan.VS_WEIGHT_F32, K=K, n=n, W=W, H=H)
+        d.max_weight = 100.0
+        return d
+
+    log(f"serving: {len(reqs)} sessions over {seconds:.0f} s, batched slot table")
+    loop = serving.ServeLoop(serving.BatchedBackend(ctx, 128, 200 << 30), desc16, ptr, frames, pose0)
+    res = loop.run(reqs, seconds, log=log)
+    loop.close()
+    res["definition"] = ("latency = completion of the step that fused the frame (GPU event) - frame arrival; "
+                         "sessions arrive Poisson 2/s after 16 at t=0, 30 fps +-5 ms, 150-900 frames, volumes "
+                         "{128,256,384,512}^3 FP16+u8, 200 GB budget, 128 slots")
+    log(f"  batched: {res['fused_frames_per_s']} fused frames/s, p50 {res['latency_ms']['p50']} ms, "
+        f"p99 {res['latency_ms']['p99']} ms")
+    if with_plain:
+        reqs = serving.make_requests(7, seconds, len(bufs))
+        log("serving: same trace, plain serial pipeline")
+        loop = serving.ServeLoop(serving.SerialBackend(ctx, 128, 200 << 30), desc32, ptr, frames, pose0,
+                                 max_batch=1)
+        plain = loop.run(reqs, seconds, log=log)
+        loop.close()
+        log(f"  plain: {plain['fused_frames_per_s']} fused frames/s, p50 {plain['latency_ms']['p50']} ms, "
+            f"p99 {plain['latency_ms']['p99']} ms")
+        res["plain_serial_fp32"] = plain
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +186,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the serving-form run (0 = skip)")
+    ap.add_argument("--serve-plain", type=int, default=1,
+                    help="also run the same request trace through the plain serial pipeline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build shards sessions across GPUs only by running one process per "
@@ -222,6 +271,11 @@ def main():
                                     "bytes": int(v[0])} for k, v in cand.items()}
         upd = sum(s.integrate_counts()[1] for s in sessions)
         out["voxel_updates_per_s"] = round(upd / (acc.sum() * 1e-3), 1)
+    if args.serve_seconds > 0:
+        for s in sessions:  # the serving run opens its own sessions
+            s.close()
+        sessions = []
+        out["serving"] = serve(ctx, streams, bufs, frames, W, H, args.serve_seconds, args.serve_plain)
     if not args.no_plain:
         log("plain FP32 one-session-at-a-time pipeline")
         for s in sessions:  # free the FP16 volumes first

@@ -184,6 +184,37 @@ vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long
  * the raycast's algorithmic bytes; no counterpart in the reference. */
 vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]);
 
+/* ---- serving: slot table with continuous batching ------------------------
+ * No counterpart in the reference (one scan per process there). A server owns
+ * up to max_slots sessions of one storage form. vs_server_open admits a
+ * session when a slot is free and the bytes of all open sessions stay within
+ * hbm_budget (0 = no limit); otherwise it returns VS_BUDGET and allocates
+ * nothing. vs_server_step fuses one frame for each listed slot in one
+ * graph-captured launch sequence; the other open slots keep their volume, pose
+ * and maps untouched. Opening or closing a slot waits for the server's steps
+ * in flight, then rebuilds the launch sequence once; steps between changes
+ * replay it with a per-step active mask. Each session's result is the same as
+ * vs_fuse_frame on it alone. Sessions stay usable through vs_server_session
+ * (pose, maps, volume download); closing destroys them. */
+typedef struct vs_server vs_server;
+typedef struct {
+    int max_slots;          /* 1..256 */
+    int tsdf_type;          /* vs_tsdf_type of every session */
+    int weight_type;        /* vs_weight_type of every session */
+    unsigned long long hbm_budget; /* bytes (vs_session_bytes summed), 0 = no limit */
+} vs_server_desc;
+vs_status vs_server_create(vs_ctx *ctx, const vs_server_desc *desc, vs_server **out);
+vs_status vs_server_destroy(vs_server *srv);
+vs_status vs_server_open(vs_server *srv, const vs_session_desc *desc, const double T_gc0[16], int *slot);
+vs_status vs_server_close(vs_server *srv, int slot);
+vs_session *vs_server_session(vs_server *srv, int slot);
+/* depths[i] is slot slots[i]'s frame (device pointer unless is_host); each
+ * slot at most once per step. Asynchronous on `stream` (NULL: the server's). */
+vs_status vs_server_step(vs_server *srv, const int *slots, const uint16_t *const *depths, int n, int is_host,
+                         vs_stream stream);
+/* Open slots and the session bytes they hold. */
+vs_status vs_server_usage(vs_server *srv, int *open_slots, unsigned long long *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,6 +145,7 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<vs_session *> members;
+    const int *active = nullptr;  // slot table: device flags of the slots with a frame this step
 };
 
 // Events around the stages of one frame step:
@@ -173,6 +174,7 @@ struct vs_ctx {
     int device = 0;
     std::map<std::vector<vs_session *>, std::unique_ptr<Batch>> batches;
     std::vector<vs_session *> sessions;
+    std::vector<vs_server *> servers;
     bool use_graph = true;
     bool timing = false;
     Timer timer;
@@ -190,6 +192,31 @@ struct vs_session {
     Batch self;
     bool timing = false;
     Timer timer;
+    vs_server *server = nullptr;  // slot table holding this session, if any
+    int slot = -1;
+};
+
+// Slot table of the serving form. B spans all max_slots slots: B.dP[i] holds
+// the parameters of the session in slot i, or for a free slot a zero
+// parameter block (no levels, no tiles) whose state pointer is that slot's
+// own placeholder state, so every kernel of a step sees an inactive session
+// and returns before touching any buffer. Work lists and the brick list cover
+// the open slots only and are rebuilt when a slot opens or closes.
+struct vs_server {
+    vs_ctx *ctx = nullptr;
+    vs_server_desc d{};
+    std::vector<vs_session *> slot;
+    hipStream_t stream = nullptr;
+    hipStream_t last = nullptr;
+    hipEvent_t order = nullptr;
+    Batch B;
+    char *fixed = nullptr;   // params[max_slots], placeholder states, active flags
+    SessState *ghost = nullptr;
+    int *active = nullptr;
+    char *lists = nullptr;   // work lists, brick list, brick counter
+    size_t lists_bytes = 0;
+    bool dirty = true;
+    unsigned long long bytes = 0;
 };
 
 namespace {
@@ -213,6 +240,94 @@ void drop_batches_with(vs_ctx *ctx, vs_session *s) {
     }
 }
 
+// Waits for the server's steps in flight (its lists and parameter array are
+// read by them).
+vs_status server_quiesce(vs_server *v) {
+    if (v->last && v->last != v->stream) HIP_TRY(hipStreamSynchronize(v->last));
+    HIP_TRY(hipStreamSynchronize(v->stream));
+    return VS_OK;
+}
+
+// Parameter block of slot i: the open session's, or the inactive placeholder.
+vs_status server_write_slot(vs_server *v, int i) {
+    SessParams p{};
+    if (v->slot[i]) {
+        p = v->slot[i]->hp;
+    } else {
+        p.st = v->ghost + i;
+    }
+    HIP_TRY(hipMemcpy(v->B.dP + i, &p, sizeof(p), hipMemcpyHostToDevice));
+    return VS_OK;
+}
+
+// Removes a session from its slot (the session itself stays alive).
+vs_status server_forget(vs_session *s) {
+    vs_server *v = s->server;
+    if (!v) return VS_OK;
+    vs_status r = server_quiesce(v);
+    const int i = s->slot;
+    v->slot[i] = nullptr;
+    v->bytes -= s->L.total;
+    s->server = nullptr;
+    s->slot = -1;
+    v->dirty = true;
+    vs_status w = server_write_slot(v, i);
+    return r != VS_OK ? r : w;
+}
+
+// Work lists (slot, tile) of the open slots, the brick list sized for their
+// bricks, and the iteration bounds of the launch sequence; drops the captured
+// graph so the next step captures the new sequence.
+vs_status server_rebuild(vs_server *v) {
+    Batch &B = v->B;
+    free_batch_graph(B);
+    std::vector<WorkItem> wk[kMaxLevels];
+    uint32_t cap = 0;
+    B.levels = 0;
+    for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = 0;
+    for (int i = 0; i < (int)v->slot.size(); ++i) {
+        const vs_session *s = v->slot[i];
+        if (!s) continue;
+        B.levels = std::max(B.levels, s->d.levels);
+        for (int l = 0; l < kMaxLevels; ++l) {
+            B.max_iters[l] = std::max(B.max_iters[l], s->hp.iters[l]);
+            for (int t = 0; t < s->L.tiles[l]; ++t) wk[l].push_back({i, t});
+        }
+        cap += s->L.nbricks;
+    }
+    size_t off = 0, offW[kMaxLevels];
+    for (int l = 0; l < kMaxLevels; ++l) {
+        offW[l] = off;
+        off = align_up(off + std::max<size_t>(wk[l].size(), 1) * sizeof(WorkItem));
+    }
+    const size_t offB = off;
+    off = align_up(off + (size_t)std::max<uint32_t>(cap, 1) * sizeof(int2));
+    const size_t offC = off;
+    off = align_up(off + 16);
+    if (off > v->lists_bytes) {
+        if (v->lists) HIP_TRY(hipFree(v->lists));
+        v->lists = nullptr;
+        v->lists_bytes = 0;
+        const size_t want = off + off / 2;
+        hipError_t e = hipMalloc((void **)&v->lists, want);
+        if (e != hipSuccess) return fail(VS_ENOMEM, "server lists (%zu bytes): %s", want, hipGetErrorString(e));
+        v->lists_bytes = want;
+    }
+    for (int l = 0; l < kMaxLevels; ++l) {
+        B.dWork[l] = (WorkItem *)(v->lists + offW[l]);
+        B.nWork[l] = (int)wk[l].size();
+        if (!wk[l].empty())
+            HIP_TRY(hipMemcpy(B.dWork[l], wk[l].data(), wk[l].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    }
+    B.bl.items = (int2 *)(v->lists + offB);
+    B.bl.count = (unsigned *)(v->lists + offC);
+    B.bl.capacity = cap;
+    HIP_TRY(hipMemset(B.bl.count, 0, 16));
+    B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(cap, 8192));
+    v->dirty = false;
+    return VS_OK;
+}
+
 hipStream_t pick(vs_session *s, vs_stream st) {
     hipStream_t h = st ? (hipStream_t)st : s->stream;
     s->last = h;
@@ -225,7 +340,10 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
         if (timed) (void)hipEventRecord(timed->ev[i], s);
     };
     mark(0);
-    launch_track_setup(B.dP, B.n, nullptr, s);
+    if (B.active)
+        launch_track_setup_active(B.dP, B.n, B.active, s);
+    else
+        launch_track_setup(B.dP, B.n, nullptr, s);
     launch_bilateral(B.dP, B.dWork[0], B.nWork[0], s);
     mark(1);
     for (int l = 1; l < B.levels; ++l) launch_pyr_down(B.dP, B.dWork[l], B.nWork[l], l, s);
@@ -343,6 +461,8 @@ vs_status vs_ctx_create(int device, vs_ctx **out) {
 vs_status vs_ctx_destroy(vs_ctx *ctx) {
     if (!ctx) return VS_OK;
     hipSetDevice(ctx->device);
+    std::vector<vs_server *> vv = ctx->servers;
+    for (vs_server *v : vv) vs_server_destroy(v);
     for (auto &kv : ctx->batches) {
         free_batch_graph(*kv.second);
         if (kv.second->owned) hipFree(kv.second->owned);
@@ -494,7 +614,8 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
 vs_status vs_session_destroy(vs_session *s) {
     if (!s) return VS_OK;
     hipSetDevice(s->ctx->device);
-    hipStreamSynchronize(s->stream);
+    server_forget(s);
+    (void)sync_last(s);
     drop_batches_with(s->ctx, s);
     free_batch_graph(s->self);
     s->timer.destroy();
@@ -513,6 +634,7 @@ vs_status vs_session_reset(vs_session *s, const double T_gc0[16], vs_stream stre
     SessState init{};
     for (int k = 0; k < 16; ++k) init.T_gc[k] = T_gc0 ? T_gc0[k] : ((k % 5) == 0 ? 1.0 : 0.0);
     init.conv_level = -1;
+    init.active = 1;
     HIP_TRY(hipMemcpyAsync(s->st, &init, sizeof(init), hipMemcpyHostToDevice, h));
     launch_pose_derived(s->self.dP, h);
     HIP_TRY(hipGetLastError());
@@ -539,6 +661,7 @@ vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_
 vs_status vs_bilateral(vs_session *s, vs_stream stream) {
     if (!s) return fail(VS_EINVAL, "null session");
     hipStream_t h = pick(s, stream);
+    launch_fuse_ok(s->self.dP, 1, h);
     launch_bilateral(s->self.dP, s->self.dWork[0], s->self.nWork[0], h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
@@ -547,6 +670,7 @@ vs_status vs_bilateral(vs_session *s, vs_stream stream) {
 vs_status vs_pyramid(vs_session *s, vs_stream stream) {
     if (!s) return fail(VS_EINVAL, "null session");
     hipStream_t h = pick(s, stream);
+    launch_fuse_ok(s->self.dP, 1, h);
     for (int l = 1; l < s->d.levels; ++l) launch_pyr_down(s->self.dP, s->self.dWork[l], s->self.nWork[l], l, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
@@ -676,11 +800,11 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
         // device-resident frames: one gather launch per 64 sessions
         for (int first = 0; first < n; first += kDepthChunk) {
             DepthChunk ch{};
-            ch.first = first;
             ch.n = std::min(kDepthChunk, n - first);
             for (int i = 0; i < ch.n; ++i) {
                 if (!depths[first + i]) return fail(VS_EINVAL, "null depth for session %d", first + i);
                 ch.src[i] = depths[first + i];
+                ch.slot[i] = first + i;
             }
             launch_gather_depth(B->dP, ch, h);
         }
@@ -920,6 +1044,169 @@ vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy(counts, d, 24, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// serving: slot table
+// ---------------------------------------------------------------------------
+vs_status vs_server_create(vs_ctx *ctx, const vs_server_desc *desc, vs_server **out) {
+    if (!ctx || !desc || !out) return fail(VS_EINVAL, "null argument");
+    if (desc->max_slots < 1 || desc->max_slots > kMaxSlots)
+        return fail(VS_EINVAL, "max_slots %d outside 1..%d", desc->max_slots, kMaxSlots);
+    if (desc->tsdf_type < 0 || desc->tsdf_type > 1 || desc->weight_type < 0 || desc->weight_type > 1)
+        return fail(VS_EINVAL, "storage type");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<vs_server> v(new vs_server());
+    v->ctx = ctx;
+    v->d = *desc;
+    const int n = desc->max_slots;
+    v->slot.assign(n, nullptr);
+    const size_t offP = 0, offG = align_up(n * sizeof(SessParams)), offA = offG + align_up(n * sizeof(SessState));
+    const size_t total = offA + align_up(n * sizeof(int));
+    hipError_t e = hipMalloc((void **)&v->fixed, total);
+    if (e != hipSuccess) return fail(VS_ENOMEM, "server tables (%zu bytes): %s", total, hipGetErrorString(e));
+    HIP_TRY(hipMemset(v->fixed, 0, total));
+    v->B.n = n;
+    v->B.tt = desc->tsdf_type;
+    v->B.wt = desc->weight_type;
+    v->B.dP = (SessParams *)(v->fixed + offP);
+    v->ghost = (SessState *)(v->fixed + offG);
+    v->active = (int *)(v->fixed + offA);
+    v->B.active = v->active;
+    HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&v->order, hipEventDisableTiming));
+    v->last = v->stream;
+    for (int i = 0; i < n; ++i) {
+        vs_status r = server_write_slot(v.get(), i);
+        if (r != VS_OK) {
+            hipFree(v->fixed);
+            return r;
+        }
+    }
+    ctx->servers.push_back(v.get());
+    *out = v.release();
+    return VS_OK;
+}
+
+vs_status vs_server_destroy(vs_server *v) {
+    if (!v) return VS_OK;
+    hipSetDevice(v->ctx->device);
+    (void)server_quiesce(v);
+    for (int i = 0; i < (int)v->slot.size(); ++i)
+        if (v->slot[i]) vs_session_destroy(v->slot[i]);
+    free_batch_graph(v->B);
+    if (v->lists) hipFree(v->lists);
+    if (v->fixed) hipFree(v->fixed);
+    if (v->order) hipEventDestroy(v->order);
+    if (v->stream) hipStreamDestroy(v->stream);
+    auto &vv = v->ctx->servers;
+    vv.erase(std::remove(vv.begin(), vv.end(), v), vv.end());
+    delete v;
+    return VS_OK;
+}
+
+vs_status vs_server_open(vs_server *v, const vs_session_desc *desc, const double T_gc0[16], int *slot) {
+    if (!v || !desc || !slot) return fail(VS_EINVAL, "null argument");
+    vs_status cs = check_desc(desc);
+    if (cs != VS_OK) return cs;
+    if (desc->tsdf_type != v->d.tsdf_type || desc->weight_type != v->d.weight_type)
+        return fail(VS_EINVAL, "session storage types differ from the server's");
+    const unsigned long long need = make_layout(*desc).total;
+    if (v->d.hbm_budget && v->bytes + need > v->d.hbm_budget)
+        return fail(VS_BUDGET, "session needs %llu bytes; %llu of %llu in use", need, v->bytes, v->d.hbm_budget);
+    int i = 0;
+    while (i < (int)v->slot.size() && v->slot[i]) ++i;
+    if (i == (int)v->slot.size()) return fail(VS_BUDGET, "all %d slots open", (int)v->slot.size());
+    vs_session *s = nullptr;
+    vs_status r = vs_session_create(v->ctx, desc, T_gc0, &s);
+    if (r != VS_OK) return r;
+    r = server_quiesce(v);
+    if (r != VS_OK) {
+        vs_session_destroy(s);
+        return r;
+    }
+    v->slot[i] = s;
+    s->server = v;
+    s->slot = i;
+    v->bytes += s->L.total;
+    v->dirty = true;
+    r = server_write_slot(v, i);
+    if (r != VS_OK) {
+        vs_session_destroy(s);
+        return r;
+    }
+    *slot = i;
+    return VS_OK;
+}
+
+vs_status vs_server_close(vs_server *v, int slot) {
+    if (!v) return fail(VS_EINVAL, "null server");
+    if (slot < 0 || slot >= (int)v->slot.size() || !v->slot[slot]) return fail(VS_EINVAL, "slot %d not open", slot);
+    return vs_session_destroy(v->slot[slot]);
+}
+
+vs_session *vs_server_session(vs_server *v, int slot) {
+    if (!v || slot < 0 || slot >= (int)v->slot.size()) return nullptr;
+    return v->slot[slot];
+}
+
+vs_status vs_server_step(vs_server *v, const int *slots, const uint16_t *const *depths, int n, int is_host,
+                         vs_stream stream) {
+    if (!v || !slots || !depths || n <= 0) return fail(VS_EINVAL, "bad step arguments");
+    const int ns = (int)v->slot.size();
+    if (n > ns) return fail(VS_EINVAL, "%d frames for %d slots", n, ns);
+    uint32_t bits[kMaxSlots / 32] = {};
+    for (int k = 0; k < n; ++k) {
+        const int i = slots[k];
+        if (i < 0 || i >= ns || !v->slot[i]) return fail(VS_EINVAL, "slot %d not open", i);
+        if ((bits[i >> 5] >> (i & 31)) & 1u) return fail(VS_EINVAL, "slot %d repeated in step", i);
+        if (!depths[k]) return fail(VS_EINVAL, "null depth for slot %d", i);
+        bits[i >> 5] |= 1u << (i & 31);
+    }
+    HIP_TRY(hipSetDevice(v->ctx->device));
+    if (v->dirty) {
+        vs_status r = server_quiesce(v);
+        if (r == VS_OK) r = server_rebuild(v);
+        if (r != VS_OK) return r;
+    }
+    hipStream_t h = stream ? (hipStream_t)stream : v->stream;
+    if (h != v->last) {
+        // steps share the slot table's lists: order this one after the previous
+        HIP_TRY(hipEventRecord(v->order, v->last));
+        HIP_TRY(hipStreamWaitEvent(h, v->order, 0));
+    }
+    if (is_host) {
+        for (int k = 0; k < n; ++k) {
+            vs_status r = copy_depth(v->slot[slots[k]], depths[k], 1, h);
+            if (r != VS_OK) return r;
+        }
+    } else {
+        for (int first = 0; first < n; first += kDepthChunk) {
+            DepthChunk ch{};
+            ch.n = std::min(kDepthChunk, n - first);
+            for (int k = 0; k < ch.n; ++k) {
+                ch.src[k] = depths[first + k];
+                ch.slot[k] = slots[first + k];
+            }
+            launch_gather_depth(v->B.dP, ch, h);
+        }
+    }
+    launch_set_active(v->active, ns, bits, h);
+    HIP_TRY(hipGetLastError());
+    for (int k = 0; k < n; ++k) v->slot[slots[k]]->last = h;
+    v->last = h;
+    return run_step(v->ctx, v->B, h, v->ctx->timing ? &v->ctx->timer : nullptr);
+}
+
+vs_status vs_server_usage(vs_server *v, int *open_slots, unsigned long long *bytes) {
+    if (!v) return fail(VS_EINVAL, "null server");
+    if (open_slots) {
+        int c = 0;
+        for (vs_session *s : v->slot) c += s != nullptr;
+        *open_slots = c;
+    }
+    if (bytes) *bytes = v->bytes;
     return VS_OK;
 }
 

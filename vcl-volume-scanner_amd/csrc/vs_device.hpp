@@ -45,6 +45,7 @@ struct alignas(16) SessState {
     float dist_cc;     // |cube centre - camera|
     float cos_max;     // raycast cone threshold
     int frames;        // frames fused so far
+    int active;        // the session has a frame in the current step
     int do_track;      // this frame runs the tracker
     int icp_failed;    // 0 running, else a vs_status code
     int conv_level;    // level whose iterations converged (-1 none)
@@ -102,6 +103,10 @@ struct WorkItem {
     int sess;  // index into the SessParams array of the launch
     int tile;  // tile (image stages) within the session's level
 };
+
+// A work item beyond its session's tile count (slot tables keep work lists
+// sized for the largest frame so their launch grid never changes).
+__device__ __forceinline__ bool tile_out(const LevelDesc &L, int tile) { return tile >= L.tiles_x * L.tiles_y; }
 
 // ---------------------------------------------------------------------------
 // FP32 helpers, reference operation order

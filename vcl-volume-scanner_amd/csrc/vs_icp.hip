@@ -234,10 +234,11 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
-    if (icp_skip(P, st, level, iter)) return;
+    if (level >= P.levels || icp_skip(P, st, level, iter)) return;
+    const LevelDesc &L = P.lev[level];
+    if (tile_out(L, wi.tile)) return;
     if (threadIdx.x < 16) T21s[threadIdx.x] = st->T_21_f[threadIdx.x];
     __syncthreads();
-    const LevelDesc &L = P.lev[level];
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     float A[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
@@ -315,11 +316,19 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
 }
 
 // Per-frame / per-track reset of the tracker state (one thread per session).
+// `active` (nullable: all active) marks the sessions of a slot table that have
+// a frame in this step; the others keep their state untouched by the step.
 __global__ void k_track_setup(const SessParams *__restrict__ SP, int nsess, Mat16d Tinit, int has_init,
-                              int force_track) {
+                              int force_track, const int *__restrict__ active) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsess) return;
     SessState *st = SP[s].st;
+    st->active = active ? active[s] : 1;
+    if (!st->active) {
+        st->do_track = 0;
+        st->fuse_ok = 0;
+        return;
+    }
     st->do_track = force_track ? 1 : (st->frames > 0);
     st->icp_failed = 0;
     st->conv_level = -1;
@@ -364,6 +373,10 @@ __global__ void k_track_finalize(const SessParams *__restrict__ SP, int nsess) {
     if (s >= nsess) return;
     const SessParams &P = SP[s];
     SessState *st = P.st;
+    if (!st->active) {
+        st->fuse_ok = 0;
+        return;
+    }
     int status = 0;
     if (st->do_track) {
         if (st->icp_failed) {
@@ -415,7 +428,10 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_system(const SessParams *_
 // Standalone integrate/raycast calls always run: mark the frame fusable.
 __global__ void k_fuse_ok(const SessParams *__restrict__ SP, int nsess) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nsess) SP[s].st->fuse_ok = 1;
+    if (s < nsess) {
+        SP[s].st->fuse_ok = 1;
+        SP[s].st->active = 1;
+    }
 }
 void launch_fuse_ok(const SessParams *P, int nsess, hipStream_t s) {
     hipLaunchKernelGGL(k_fuse_ok, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess);
@@ -427,14 +443,35 @@ void launch_track_setup(const SessParams *P, int nsess, const double *T_init, hi
     if (has)
         for (int k = 0; k < 16; ++k) m.m[k] = T_init[k];
     // force_track: an explicit T_init call (vs_track) always tracks
-    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 0);
+    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 0,
+                       (const int *)nullptr);
 }
 void launch_track_setup_forced(const SessParams *P, int nsess, const double *T_init, hipStream_t s) {
     Mat16d m{};
     int has = T_init != nullptr;
     if (has)
         for (int k = 0; k < 16; ++k) m.m[k] = T_init[k];
-    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 1);
+    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, has, 1,
+                       (const int *)nullptr);
+}
+void launch_track_setup_active(const SessParams *P, int nsess, const int *active, hipStream_t s) {
+    Mat16d m{};
+    hipLaunchKernelGGL(k_track_setup, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess, m, 0, 0, active);
+}
+
+// Writes a slot table's active flags from a bit mask passed by value, so a
+// captured step reads them from device memory.
+struct ActiveMask {
+    uint32_t bits[kMaxSlots / 32];
+};
+__global__ void k_set_active(int *active, int nslots, ActiveMask m) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslots) active[s] = (m.bits[s >> 5] >> (s & 31)) & 1u;
+}
+void launch_set_active(int *active, int nslots, const uint32_t bits[kMaxSlots / 32], hipStream_t s) {
+    ActiveMask m;
+    for (int k = 0; k < kMaxSlots / 32; ++k) m.bits[k] = bits[k];
+    hipLaunchKernelGGL(k_set_active, dim3((nslots + 255) / 256), dim3(256), 0, s, active, nslots, m);
 }
 void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess, int level, int iter,
                      hipStream_t s) {

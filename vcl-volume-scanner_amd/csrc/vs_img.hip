@@ -24,6 +24,7 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const LevelDesc &L = P.lev[0];
+    if (tile_out(L, wi.tile) || !P.st->active) return;
     const int W = L.W, H = L.H;
     const int x0 = (wi.tile % L.tiles_x) * kTileW, y0 = (wi.tile / L.tiles_x) * kTileH;
     const int R = P.bil_radius;
@@ -66,6 +67,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pyr_down(const SessParams *__r
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const LevelDesc &L = P.lev[l];
+    if (l >= P.levels || tile_out(L, wi.tile) || !P.st->active) return;
     const int Wi = P.lev[l - 1].W, Hi = P.lev[l - 1].H;
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(kTileThreads) void k_vnmap(const SessParams *__rest
 __global__ __launch_bounds__(256) void k_gather_depth(const SessParams *__restrict__ SP, DepthChunk ch) {
     const int s = blockIdx.y;
     if (s >= ch.n) return;
-    const SessParams &P = SP[ch.first + s];
+    const SessParams &P = SP[ch.slot[s]];
     const size_t npx = (size_t)P.lev[0].W * P.lev[0].H;
     const uint16_t *src = ch.src[s];
     uint16_t *dst = const_cast<uint16_t *>(P.raw);

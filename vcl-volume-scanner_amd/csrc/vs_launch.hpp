@@ -9,9 +9,10 @@ namespace vs {
 
 // image stages (vs_img.hip)
 constexpr int kDepthChunk = 64;
-struct DepthChunk {  // passed by value: device frame pointers of sessions [first, first+n)
+struct DepthChunk {  // passed by value: device frame pointers and the batch slots they go to
     const uint16_t *src[kDepthChunk];
-    int first, n;
+    int slot[kDepthChunk];
+    int n;
 };
 void launch_gather_depth(const SessParams *P, const DepthChunk &ch, hipStream_t s);
 void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s);
@@ -20,6 +21,10 @@ void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int level, f
 
 // tracker (vs_icp.hip); nsess = sessions in the batch
 void launch_track_setup(const SessParams *P, int nsess, const double *T_init, hipStream_t s);
+// slot-table form: active[s] != 0 selects the slots that fuse this step
+constexpr int kMaxSlots = 256;
+void launch_track_setup_active(const SessParams *P, int nsess, const int *active, hipStream_t s);
+void launch_set_active(int *active, int nslots, const uint32_t bits[kMaxSlots / 32], hipStream_t s);
 void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess, int level, int iter,
                      hipStream_t s);
 void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s);

@@ -24,6 +24,7 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
     const int s = blockIdx.y;
     if (s >= nsess) return;
     const SessParams &P = SP[s];
+    if (!P.st->active) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.st->vis_count = 0u;
         P.st->upd_count = 0ull;
@@ -59,10 +60,10 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
-    if (!st->fuse_ok) return;
+    const LevelDesc &L = P.lev[0];
+    if (!st->fuse_ok || tile_out(L, wi.tile)) return;
     for (int i = threadIdx.x; i < kVisSet; i += kTileThreads) set[i] = -1;
     __syncthreads();
-    const LevelDesc &L = P.lev[0];
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     if (x >= L.W || y >= L.H) return;
@@ -351,13 +352,13 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
-    if (!st->fuse_ok) return;
+    const LevelDesc &L = P.lev[level];
+    if (!st->fuse_ok || level >= P.levels || tile_out(L, wi.tile)) return;
     const bool lds_bits = P.socc_words <= kSoccLdsWords;
     if (lds_bits) {
         for (int i = threadIdx.x; i < P.socc_words; i += kTileThreads) s_socc[i] = P.socc[i];
         __syncthreads();
     }
-    const LevelDesc &L = P.lev[level];
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     if (x >= L.W || y >= L.H) return;

@@ -10,6 +10,7 @@ import os
 
 _hip = None
 H2D, D2H, D2D = 1, 2, 3
+HIP_NOT_READY = 600  # hipErrorNotReady
 
 
 def hip():
@@ -19,7 +20,7 @@ def hip():
         L = C.CDLL(path, mode=C.RTLD_GLOBAL)
         for f in ["hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize", "hipStreamCreate",
                   "hipStreamDestroy", "hipStreamSynchronize", "hipEventCreate", "hipEventDestroy", "hipEventRecord",
-                  "hipEventSynchronize", "hipEventElapsedTime", "hipGetDeviceCount", "hipMemGetInfo"]:
+                  "hipEventSynchronize", "hipEventElapsedTime", "hipGetDeviceCount", "hipMemGetInfo", "hipEventQuery"]:
             getattr(L, f).restype = C.c_int
         L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.hipFree.argtypes = [C.c_void_p]
@@ -32,6 +33,7 @@ def hip():
         L.hipEventDestroy.argtypes = [C.c_void_p]
         L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         L.hipEventSynchronize.argtypes = [C.c_void_p]
+        L.hipEventQuery.argtypes = [C.c_void_p]
         L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         L.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
         L.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
@@ -89,6 +91,13 @@ class Event:
 
     def sync(self):
         _ok(hip().hipEventSynchronize(self.h), "hipEventSynchronize")
+
+    def query(self):
+        """True once the work before the event has completed."""
+        r = hip().hipEventQuery(self.h)
+        if r not in (0, HIP_NOT_READY):
+            _ok(r, "hipEventQuery")
+        return r == 0
 
     def elapsed_ms(self, later):
         ms = C.c_float()

@@ -34,6 +34,11 @@ class SessionDesc(C.Structure):
     ]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("max_slots", C.c_int), ("tsdf_type", C.c_int), ("weight_type", C.c_int),
+                ("hbm_budget", C.c_ulonglong)]
+
+
 class Prim(C.Structure):
     _fields_ = [("kind", C.c_int), ("p", C.c_double * 6)]
 
@@ -81,7 +86,9 @@ def lib():
                      "vs_fuse_frame", "vs_fuse_batch", "vs_get_frame_status", "vs_get_pose", "vs_set_pose",
                      "vs_get_filtered", "vs_set_filtered", "vs_get_maps", "vs_set_maps", "vs_volume_download",
                      "vs_volume_upload", "vs_icp_system", "vs_session_set_timing", "vs_get_stage_ms",
-                     "vs_get_integrate_counts", "vs_raycast_counts", "vs_ctx_set_timing", "vs_ctx_stage_ms"]:
+                     "vs_get_integrate_counts", "vs_raycast_counts", "vs_ctx_set_timing", "vs_ctx_stage_ms",
+                     "vs_server_create", "vs_server_destroy", "vs_server_open", "vs_server_close",
+                     "vs_server_step", "vs_server_usage"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
@@ -111,6 +118,14 @@ def lib():
         L.vs_raycast_counts.argtypes = [P, C.c_int, P]
         L.vs_ctx_set_timing.argtypes = [P, C.c_int]
         L.vs_ctx_stage_ms.argtypes = [P, P]
+        L.vs_server_create.argtypes = [P, C.POINTER(ServerDesc), C.POINTER(P)]
+        L.vs_server_destroy.argtypes = [P]
+        L.vs_server_open.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(C.c_int)]
+        L.vs_server_close.argtypes = [P, C.c_int]
+        L.vs_server_session.argtypes = [P, C.c_int]
+        L.vs_server_session.restype = P
+        L.vs_server_step.argtypes = [P, P, P, C.c_int, C.c_int, P]
+        L.vs_server_usage.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_ulonglong)]
         _lib = L
     return _lib
 
@@ -274,18 +289,22 @@ class Context:
 
 
 class Session:
-    def __init__(self, ctx, desc, T_gc0=None):
+    def __init__(self, ctx, desc, T_gc0=None, handle=None):
         self.ctx = ctx
         self.desc = desc
         self.W, self.H = desc.width, desc.height
+        self.owned = handle is None
+        if handle is not None:  # a session owned by a server slot
+            self.h = P(handle)
+            return
         self.h = P()
         T = None if T_gc0 is None else np.ascontiguousarray(T_gc0, np.float64).reshape(16)
         check(lib().vs_session_create(ctx.h, C.byref(desc), _ptr(T), C.byref(self.h)), "vs_session_create")
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             lib().vs_session_destroy(self.h)
-            self.h = P()
+        self.h = P()
 
     def level_shape(self, l):
         return (self.H >> l, self.W >> l)
@@ -405,3 +424,53 @@ class Session:
 
 def session_bytes(desc):
     return lib().vs_session_bytes(C.byref(desc))
+
+
+class Server:
+    """Slot table of the serving form (vs_server_*): sessions opened into
+    slots under an HBM budget, one frame per listed slot per step."""
+
+    def __init__(self, ctx, max_slots, tsdf=0, weight=0, hbm_budget=0):
+        self.ctx = ctx
+        self.h = P()
+        d = ServerDesc(max_slots, tsdf, weight, hbm_budget)
+        check(lib().vs_server_create(ctx.h, C.byref(d), C.byref(self.h)), "vs_server_create")
+        self.sessions = {}
+
+    def close(self):
+        if self.h:
+            lib().vs_server_destroy(self.h)
+            self.h = P()
+            self.sessions.clear()
+
+    def open(self, desc, T_gc0=None):
+        """Slot index of the new session, or None when the budget or the
+        slots are exhausted (VS_BUDGET)."""
+        T = None if T_gc0 is None else np.ascontiguousarray(T_gc0, np.float64).reshape(16)
+        slot = C.c_int(-1)
+        code = lib().vs_server_open(self.h, C.byref(desc), _ptr(T), C.byref(slot))
+        if code == VS_BUDGET:
+            return None
+        check(code, "vs_server_open")
+        self.sessions[slot.value] = Session(self.ctx, desc, handle=lib().vs_server_session(self.h, slot.value))
+        return slot.value
+
+    def close_slot(self, slot):
+        s = self.sessions.pop(slot)
+        s.h = P()
+        check(lib().vs_server_close(self.h, slot), "vs_server_close")
+
+    def session(self, slot):
+        return self.sessions[slot]
+
+    def step(self, slots, depths, is_host=True, stream=None):
+        n = len(slots)
+        sl = np.ascontiguousarray(slots, np.int32)
+        keep = [np.ascontiguousarray(d, np.uint16) if is_host else d for d in depths]
+        ds = (P * n)(*[(_ptr(k) if is_host else P(k)) for k in keep])
+        check(lib().vs_server_step(self.h, _ptr(sl), ds, n, 1 if is_host else 0, stream), "vs_server_step")
+
+    def usage(self):
+        n, b = C.c_int(), C.c_ulonglong()
+        check(lib().vs_server_usage(self.h, C.byref(n), C.byref(b)), "vs_server_usage")
+        return n.value, b.value
