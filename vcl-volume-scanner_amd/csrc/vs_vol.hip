@@ -183,38 +183,55 @@ struct Vol {
         }
         return cocc;
     }
+    __device__ __forceinline__ bool in_grid(int x, int y, int z) const {
+        return x >= 0 && x < P.vdim[0] && y >= 0 && y < P.vdim[1] && z >= 0 && z < P.vdim[2];
+    }
+    // Value of an in-grid voxel. Weight and value are loaded together (no
+    // dependency between the two loads); no weight reads as "no data" (1.0).
+    __device__ __forceinline__ bool load(size_t idx, float &out) const {
+        const float w = ld_w(Wt, idx);
+        const float f = ld_val(F, idx);
+        const bool ok = w > 0.0f;
+        out = ok ? f : 1.0f;
+        return ok;
+    }
+    // One tap with the occupancy skip (the march's nearest taps).
     __device__ __forceinline__ bool tap(int x, int y, int z, float &out) const {
         out = 1.0f;
-        if (x < 0 || x >= P.vdim[0] || y < 0 || y >= P.vdim[1] || z < 0 || z >= P.vdim[2]) return false;
+        if (!in_grid(x, y, z)) return false;
         if (!occupied(x, y, z)) return false;
-        const int b = cb;
-        const size_t idx = (size_t)b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
-        if (!(ld_w(Wt, idx) > 0.0f)) return false;
-        out = ld_val(F, idx);
-        return true;
+        return load((size_t)cb * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out);
     }
-    // d_gtvu: nearest voxel by ROUNDF
-    __device__ __forceinline__ bool nearest(f3 v, float &out) const {
+    // d_gtvu: nearest voxel by ROUNDF. Returns 1 (data), 0 (no data) or -1
+    // when the rounded position lies outside the grid on an axis along which
+    // the ray moves away or not at all: every later position of a ray that
+    // keeps stepping forward is then outside as well.
+    __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out) const {
         int x, y, z;
         out = 1.0f;
-        if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z)) return false;
-        return tap(x, y, z, out);
+        if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z)) return 0;
+        if ((x < 0 && !(rd.x > 0.0f)) || (x >= P.vdim[0] && !(rd.x < 0.0f)) || (y < 0 && !(rd.y > 0.0f)) ||
+            (y >= P.vdim[1] && !(rd.y < 0.0f)) || (z < 0 && !(rd.z > 0.0f)) || (z >= P.vdim[2] && !(rd.z < 0.0f)))
+            return -1;
+        return tap(x, y, z, out) ? 1 : 0;
     }
-    // d_gtvi: trilinear with toInt = truncation; any invalid tap -> false, 1.0
+    // d_gtvi: trilinear with toInt = truncation; any invalid tap -> false,
+    // 1.0. The 2x2x2 stencil is range-checked as a whole and its 8 values and
+    // weights are loaded together; a voxel of an empty brick has weight 0, so
+    // no occupancy lookup is needed here.
     __device__ __forceinline__ bool trilinear(f3 v, float &out) const {
         out = 1.0f;
         if (!(fabsf(v.x) < 2147483520.0f && fabsf(v.y) < 2147483520.0f && fabsf(v.z) < 2147483520.0f)) return false;
         const int ox = (int)v.x, oy = (int)v.y, oz = (int)v.z;
+        if (!in_grid(ox, oy, oz) || !in_grid(ox + 1, oy + 1, oz + 1)) return false;
         const float rx = v.x - (float)ox, ry = v.y - (float)oy, rz = v.z - (float)oz;
-        float f0, f1, f2, f3v, b0, b1, b2, b3;
-        if (!tap(ox, oy, oz, f0) || !tap(ox + 1, oy, oz, f1) || !tap(ox, oy + 1, oz, f2) ||
-            !tap(ox + 1, oy + 1, oz, f3v))
-            return false;
-        if (!tap(ox, oy, oz + 1, b0) || !tap(ox + 1, oy, oz + 1, b1) || !tap(ox, oy + 1, oz + 1, b2) ||
-            !tap(ox + 1, oy + 1, oz + 1, b3))
-            return false;
-        const float fi = (1.0f - ry) * ((1.0f - rx) * f0 + rx * f1) + ry * ((1.0f - rx) * f2 + rx * f3v);
-        const float bi = (1.0f - ry) * ((1.0f - rx) * b0 + rx * b1) + ry * ((1.0f - rx) * b2 + rx * b3);
+        float f[8];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ok &= load(vox_index(P, ox + (k & 1), oy + ((k >> 1) & 1), oz + (k >> 2)), f[k]);
+        if (!ok) return false;
+        const float fi = (1.0f - ry) * ((1.0f - rx) * f[0] + rx * f[1]) + ry * ((1.0f - rx) * f[2] + rx * f[3]);
+        const float bi = (1.0f - ry) * ((1.0f - rx) * f[4] + rx * f[5]) + ry * ((1.0f - rx) * f[6] + rx * f[7]);
         out = (1.0f - rz) * fi + rz * bi;
         return true;
     }
@@ -239,7 +256,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
     world_to_voxel(V.P, ps, vox);
     float tsdf;
     if (STATS) ++n_near;
-    const bool valid = V.nearest(vox, tsdf);
+    const bool valid = V.nearest(vox, rd, tsdf) > 0;
     bool coarse;
     if (!valid) coarse = true;
     else if (tsdf >= 0.0f) coarse = false;
@@ -255,7 +272,10 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         total += step;
         if (total > C.total_max) return false;
         if (STATS) ++n_near;
-        V.nearest(vox, tsdf);
+        // A ray that has left the grid for good can only end as a miss
+        // (every later tap reads 1.0 and the step stays positive); the
+        // instrumented form keeps marching to count the reference's taps.
+        if (V.nearest(vox, rd, tsdf) < 0 && !STATS) return false;
         if (tsdf > -1.0f && tsdf < 1.0f) {
             if (STATS) ++n_tri;
             V.trilinear(vox, tsdf);
@@ -294,19 +314,21 @@ __device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n
     const int dx1[4] = {0, 1, 0, 1}, dy1[4] = {0, 0, 1, 1};
     const int dx2[12] = {0, 1, -1, 0, 1, 2, -1, 0, 1, 2, 0, 1};
     const int dy2[12] = {-1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2};
+    // every tap lies in [o-1, o+2]^3 and each of those extremes is used, so
+    // the stencil is valid only if that box is inside the grid; the 32 taps
+    // are then loaded without per-tap branches (all-or-nothing result)
+    if (!V.in_grid(ox - 1, oy - 1, oz - 1) || !V.in_grid(ox + 2, oy + 2, oz + 2)) return false;
     float fm[4], fr[12], bk[12], bm[4];
+    bool ok = true;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (!V.tap(ox + dx1[i], oy + dy1[i], oz - 1, fm[i])) return false;
+    for (int i = 0; i < 4; ++i) ok &= V.load(vox_index(V.P, ox + dx1[i], oy + dy1[i], oz - 1), fm[i]);
 #pragma unroll
-    for (int i = 0; i < 12; ++i)
-        if (!V.tap(ox + dx2[i], oy + dy2[i], oz, fr[i])) return false;
+    for (int i = 0; i < 12; ++i) ok &= V.load(vox_index(V.P, ox + dx2[i], oy + dy2[i], oz), fr[i]);
 #pragma unroll
-    for (int i = 0; i < 12; ++i)
-        if (!V.tap(ox + dx2[i], oy + dy2[i], oz + 1, bk[i])) return false;
+    for (int i = 0; i < 12; ++i) ok &= V.load(vox_index(V.P, ox + dx2[i], oy + dy2[i], oz + 1), bk[i]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (!V.tap(ox + dx1[i], oy + dy1[i], oz + 2, bm[i])) return false;
+    for (int i = 0; i < 4; ++i) ok &= V.load(vox_index(V.P, ox + dx1[i], oy + dy1[i], oz + 2), bm[i]);
+    if (!ok) return false;
     auto Lz = [&](int i) { return (1.0f - rz) * fr[i] + rz * bk[i]; };
     float a4[4], b4[4], a2[2], b2[2];
     a4[0] = Lz(2); a4[1] = Lz(3); a4[2] = Lz(4); a4[3] = Lz(5);
@@ -344,6 +366,12 @@ __device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n
     return true;
 }
 
+// The raycast runs as two launches per level. k_raycast marches every ray
+// (g_rmi's d_crtc part) with a small register footprint, so 8 waves per SIMD
+// hide the march's dependent taps; a hit leaves its world point in the vertex
+// map with w = 1, a miss writes its final sentinels. k_raycast_shade then
+// evaluates the 32-tap gradient of every hit with all its loads in flight at
+// once and writes the final vertex / normal / shade (camera frame).
 template <class TV, class TW, bool STATS>
 __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__restrict__ SP,
                                                           const WorkItem *__restrict__ work, int level,
@@ -375,11 +403,41 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
     C.max_iter = P.max_iter;
     const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, lds_bits ? s_socc : P.socc};
     unsigned n_near = 0, n_tri = 0;
-    f3 p, n;
-    const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri) &&
-                    surface_normal(V, p, n);
+    f3 p;
+    const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
+        P.mvtx[level][t] = make_float4(p.x, p.y, p.z, 1.0f);
+    } else {
+        P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
+        P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
+        P.mimg[level][t] = 0;
+    }
+    if (STATS) {
+        atomicAdd(stats + 0, (unsigned long long)n_near);
+        atomicAdd(stats + 1, (unsigned long long)n_tri);
+    }
+}
+
+template <class TV, class TW, bool STATS>
+__global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams *__restrict__ SP,
+                                                                const WorkItem *__restrict__ work, int level,
+                                                                unsigned long long *stats) {
+    const WorkItem wi = work[blockIdx.x];
+    const SessParams &P = SP[wi.sess];
+    const SessState *st = P.st;
+    const LevelDesc &L = P.lev[level];
+    if (!st->fuse_ok || level >= P.levels || tile_out(L, wi.tile)) return;
+    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    if (x >= L.W || y >= L.H) return;
+    const size_t t = (size_t)y * L.W + x;
+    const float4 h = P.mvtx[level][t];
+    if (h.w != 1.0f) return;  // a miss: final already
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, P.socc};
+    const f3 p = {h.x, h.y, h.z};
+    f3 n;
+    if (surface_normal(V, p, n)) {
         const float *T = st->T_gc_f;
         const f3 pc = transform(T, p);
         const f3 nc = {T[0] * n.x + T[1] * n.y + T[2] * n.z, T[4] * n.x + T[5] * n.y + T[6] * n.z,
@@ -389,15 +447,11 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
         // light (0,0,-1): n.l = 0*nx + 0*ny - nz (z_yooji_object_scanner.cpp:1313)
         const float sa = nc.x * 0.0f + nc.y * 0.0f + nc.z * -1.0f;
         P.mimg[level][t] = (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f)));
+        if (STATS) atomicAdd(stats + 2, 1ull);
     } else {
         P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
         P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
         P.mimg[level][t] = 0;
-    }
-    if (STATS) {
-        atomicAdd(stats + 0, (unsigned long long)n_near);
-        atomicAdd(stats + 1, (unsigned long long)n_tri);
-        if (ok) atomicAdd(stats + 2, 1ull);
     }
 }
 
@@ -499,10 +553,15 @@ template <class TV, class TW>
 static void raycast_t(const SessParams *P, const WorkItem *work, int n, int level, unsigned long long *stats,
                       hipStream_t s) {
     if (n <= 0) return;
-    if (stats)
+    if (stats) {
         hipLaunchKernelGGL((k_raycast<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, work, level, stats);
-    else
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, work, level,
+                           stats);
+    } else {
         hipLaunchKernelGGL((k_raycast<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, work, level, stats);
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, work, level,
+                           stats);
+    }
 }
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
     VS_DISPATCH(tt, wt, raycast_t, P, work, n, level, nullptr, s);
