@@ -205,15 +205,36 @@ struct Vol {
     // d_gtvu: nearest voxel by ROUNDF. Returns 1 (data), 0 (no data) or -1
     // when the rounded position lies outside the grid on an axis along which
     // the ray moves away or not at all: every later position of a ray that
-    // keeps stepping forward is then outside as well.
-    __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out) const {
+    // keeps stepping forward is then outside as well. For "no data" it also
+    // reports what the lookup learnt about the neighbourhood: size > 0 when
+    // the voxel box [lo, lo + size)^3 (an empty superbrick or brick) holds no
+    // data at all, size = -1 when the position is outside the grid (moving
+    // towards it), size = 0 otherwise.
+    __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out, int3 &lo, int &size) const {
         int x, y, z;
         out = 1.0f;
+        size = 0;
         if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z)) return 0;
-        if ((x < 0 && !(rd.x > 0.0f)) || (x >= P.vdim[0] && !(rd.x < 0.0f)) || (y < 0 && !(rd.y > 0.0f)) ||
-            (y >= P.vdim[1] && !(rd.y < 0.0f)) || (z < 0 && !(rd.z > 0.0f)) || (z >= P.vdim[2] && !(rd.z < 0.0f)))
-            return -1;
-        return tap(x, y, z, out) ? 1 : 0;
+        if (!in_grid(x, y, z)) {
+            if ((x < 0 && !(rd.x > 0.0f)) || (x >= P.vdim[0] && !(rd.x < 0.0f)) || (y < 0 && !(rd.y > 0.0f)) ||
+                (y >= P.vdim[1] && !(rd.y < 0.0f)) || (z < 0 && !(rd.z > 0.0f)) ||
+                (z >= P.vdim[2] && !(rd.z < 0.0f)))
+                return -1;
+            size = -1;
+            return 0;
+        }
+        if (!occupied(x, y, z)) {
+            const int sh = csocc ? 3 : 5;  // empty brick, or a whole empty superbrick
+            lo = make_int3((x >> sh) << sh, (y >> sh) << sh, (z >> sh) << sh);
+            size = 1 << sh;
+            return 0;
+        }
+        return load((size_t)cb * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
+    }
+    __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out) const {
+        int3 lo;
+        int size;
+        return nearest(v, rd, out, lo, size);
     }
     // d_gtvi: trilinear with toInt = truncation; any invalid tap -> false,
     // 1.0. The 2x2x2 stencil is range-checked as a whole and its 8 values and
@@ -243,6 +264,47 @@ struct RayConst {
     float dist_cc, cos_max, r_cube, step_scale, total_max;
     int max_iter;
 };
+
+// Exact fast-forward of the march through a region that holds no data.
+// While every tap reads "no data" (1.0) the march repeats the same step S:
+// position += S*rd (the same FP32 product every time), total += S, iter += 1,
+// and nothing else changes. These return how many further steps certainly
+// keep the ROUNDF position inside the region: the bounds carry a 0.25-voxel
+// margin, far above the rounding drift of at most 64 FP32 additions of
+// positions below 2^13 voxels (< 0.02 voxel), so each skipped step's tap would
+// have read "no data" too.
+constexpr int kSkipMax = 64;
+constexpr float kSkipMargin = 0.25f;
+
+// rounded position inside [lo, lo + size)^3 clipped to the grid:
+// v in [lo - 0.5, hi + 0.5) on each axis
+__device__ __forceinline__ int steps_in_box(f3 v, f3 d, int3 lo, int size, const int *vdim) {
+    float k = (float)kSkipMax;
+    const float vv[3] = {v.x, v.y, v.z}, dd[3] = {d.x, d.y, d.z};
+    const int ll[3] = {lo.x, lo.y, lo.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float hi = (float)min(ll[a] + size - 1, vdim[a] - 1);
+        if (dd[a] > 0.0f) k = fminf(k, (hi + 0.5f - kSkipMargin - vv[a]) / dd[a]);
+        else if (dd[a] < 0.0f) k = fminf(k, ((float)ll[a] - 0.5f + kSkipMargin - vv[a]) / dd[a]);
+    }
+    return k >= 1.0f ? (int)k : 0;
+}
+
+// rounded position outside the grid: it stays outside while any axis does
+// (below: v <= -0.5; above: v >= vdim - 0.5)
+__device__ __forceinline__ int steps_outside(f3 v, f3 d, const int *vdim) {
+    float k = 0.0f;
+    const float vv[3] = {v.x, v.y, v.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (vv[a] <= -0.5f && dd[a] > 0.0f) k = fmaxf(k, (-0.5f - kSkipMargin - vv[a]) / dd[a]);
+        else if (vv[a] >= (float)vdim[a] - 0.5f && dd[a] < 0.0f)
+            k = fmaxf(k, ((float)vdim[a] - 0.5f + kSkipMargin - vv[a]) / dd[a]);
+    }
+    k = fminf(k, (float)kSkipMax);
+    return k >= 1.0f ? (int)k : 0;
+}
 
 template <class TV, class TW, bool STATS>
 __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C, const float *Tcg, float px,
@@ -275,7 +337,26 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         // A ray that has left the grid for good can only end as a miss
         // (every later tap reads 1.0 and the step stays positive); the
         // instrumented form keeps marching to count the reference's taps.
-        if (V.nearest(vox, rd, tsdf) < 0 && !STATS) return false;
+        int3 lo;
+        int size;
+        if (V.nearest(vox, rd, tsdf, lo, size) < 0 && !STATS) return false;
+        if (!STATS && size != 0) {
+            // no data here and in the region around: skip the steps that stay
+            // inside it (tsdf is 1.0, so the next step is the same S each time)
+            const float S = coarse ? C.step_scale : fmaxf(C.step_scale * tsdf, 1.0f);
+            const f3 d = scale(S, rd);
+            const int k = size > 0 ? steps_in_box(vox, d, lo, size, V.P.vdim) : steps_outside(vox, d, V.P.vdim);
+            if (k > 0) {
+                iter += k;
+                if (iter > C.max_iter) return false;
+                for (int j = 0; j < k; ++j) {
+                    vox = add(vox, d);
+                    total += S;
+                }
+                if (total > C.total_max) return false;
+            }
+            continue;
+        }
         if (tsdf > -1.0f && tsdf < 1.0f) {
             if (STATS) ++n_tri;
             V.trilinear(vox, tsdf);
