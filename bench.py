@@ -260,7 +260,7 @@ def main():
         stage = dict(zip(["reset_bilateral", "pyramid", "icp", "brick_visibility", "integrate", "raycast_l0",
                           "raycast_l1", "raycast_l2"], [round(float(x), 4) for x in acc]))
         out["stage_ms"] = stage
-        cand = {"k_integrate": (integ_b, acc[4]), "k_raycast(level 0)": (ray_b, acc[5])}
+        cand = {"k_integrate": (integ_b, acc[4]), "k_raycast + k_raycast_shade (level 0)": (ray_b, acc[5])}
         name = max(cand, key=lambda k: cand[k][1])
         b, ms = cand[name]
         ach = b / (ms * 1e-3) / 1e9
