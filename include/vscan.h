@@ -174,7 +174,8 @@ vs_status vs_get_stage_ms(vs_session *s, float ms[5]);
 /* Same for batches of this context (vs_fuse_batch): when enabled, steps are
  * launched eagerly with events between kernels, and vs_ctx_stage_ms returns the
  * last step's [track reset + bilateral, pyramid, ICP, brick visibility,
- * integrate, raycast L0, raycast L1, raycast L2] in ms (waits for the step). */
+ * integrate, raycast march (all levels, one launch), raycast shading (all
+ * levels, one launch), 0] in ms (waits for the step). */
 vs_status vs_ctx_set_timing(vs_ctx *ctx, int enable);
 vs_status vs_ctx_stage_ms(vs_ctx *ctx, float ms[8]);
 /* Counts from the last integrate: visible bricks and updated voxels. */

@@ -150,8 +150,8 @@ struct Batch {
 
 // Events around the stages of one frame step:
 //   [0,1) track setup + bilateral   [1,2) pyramid    [2,3) ICP + finalize
-//   [3,4) brick clear + mark        [4,5) integrate  [5,6) raycast L0
-//   [6,7) raycast L1                [7,8) raycast L2
+//   [3,4) brick clear + mark        [4,5) integrate  [5,6) raycast march, all levels
+//   [6,7) raycast shading, all levels                [7,8) empty
 constexpr int kMarks = 9;
 struct Timer {
     hipEvent_t ev[kMarks] = {};
@@ -357,10 +357,12 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     mark(4);
     launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, s);
     mark(5);
-    for (int l = 0; l < kMaxLevels; ++l) {
-        if (l < B.levels) launch_raycast(B.dP, B.dWork[l], B.nWork[l], l, B.tt, B.wt, s);
-        mark(6 + l);
-    }
+    const LevelWork lw = level_work(B.dWork, B.nWork, B.levels);
+    launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, s);
+    mark(6);
+    launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, s);
+    mark(7);
+    mark(8);
     if (timed) timed->used = true;
 }
 

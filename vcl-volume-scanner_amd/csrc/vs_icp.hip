@@ -126,35 +126,36 @@ __device__ void publish_T21(SessState *st) {
 // per-pixel summand (d_cssfp_Compute_Single_Summand_with_Forward_Params,
 // pose_tracker.cu:279-395; bilinear taps d_giv/d_gin, _yooji_cuda_math.h:365-481)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool bilinear_taps(const float4 *m, int W, float u, float v, int &x0, int &y0, float &rx,
-                                              float &ry, f3 &out) {
-    x0 = (int)u;
-    y0 = (int)v;
-    rx = u - (float)x0;
-    ry = v - (float)y0;
+// The 2x2 neighbourhood of (u, v) in a model map. A tap takes part only when
+// its weight is positive (rx > 0 for the right column, ry > 0 for the lower
+// row); a participating tap at -100 rejects the pixel. The four addresses are
+// always inside the map (an unused tap re-reads the anchor), so the taps of
+// both maps are loaded together, before any test.
+struct Quad {
+    float4 a, b, c, d;
+};
+__device__ __forceinline__ Quad load_quad(const float4 *m, size_t i0, size_t di, size_t dj) {
+    return {m[i0], m[i0 + di], m[i0 + dj], m[i0 + di + dj]};
+}
+__device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &out) {
     f3 t1{0, 0, 0}, t2{0, 0, 0}, t3{0, 0, 0}, t4{0, 0, 0};
-    const size_t i0 = (size_t)y0 * W + x0;
-    float4 a = m[i0];
-    if (a.x == -100.0f) return false;
-    t1 = {a.x, a.y, a.z};
+    bool ok = q.a.x != -100.0f;
+    t1 = {q.a.x, q.a.y, q.a.z};
     if (rx > 0.0f) {
-        a = m[i0 + 1];
-        if (a.x == -100.0f) return false;
-        t2 = {a.x, a.y, a.z};
+        ok &= q.b.x != -100.0f;
+        t2 = {q.b.x, q.b.y, q.b.z};
     }
     if (ry > 0.0f) {
-        a = m[i0 + W];
-        if (a.x == -100.0f) return false;
-        t3 = {a.x, a.y, a.z};
+        ok &= q.c.x != -100.0f;
+        t3 = {q.c.x, q.c.y, q.c.z};
     }
     if (rx > 0.0f && ry > 0.0f) {
-        a = m[i0 + W + 1];
-        if (a.x == -100.0f) return false;
-        t4 = {a.x, a.y, a.z};
+        ok &= q.d.x != -100.0f;
+        t4 = {q.d.x, q.d.y, q.d.z};
     }
     const float ix = 1.0f - rx, iy = 1.0f - ry;
     out = add(scale(iy, add(scale(ix, t1), scale(rx, t2))), scale(ry, add(scale(ix, t3), scale(rx, t4))));
-    return true;
+    return ok;
 }
 
 __device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, int y, const float *T21, float A[6],
@@ -167,13 +168,16 @@ __device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, i
     float u, v;
     project(p, L.K, u, v);
     if (!(u >= 0.0f && u <= float(W - 1) && v >= 0.0f && v <= float(H - 1))) return false;
-    int x0, y0;
-    float rx, ry;
+    const int x0 = (int)u, y0 = (int)v;
+    const float rx = u - (float)x0, ry = v - (float)y0;
+    const size_t i0 = (size_t)y0 * W + x0, di = rx > 0.0f ? 1 : 0, dj = ry > 0.0f ? (size_t)W : 0;
+    const Quad qv = load_quad(P.mvtx[l], i0, di, dj);
+    const Quad qn = load_quad(P.mnrm[l], i0, di, dj);
     f3 v0, n;
-    if (!bilinear_taps(P.mvtx[l], W, u, v, x0, y0, rx, ry, v0)) return false;
+    if (!bilerp(qv, rx, ry, v0)) return false;
     const float dx = v0.x - p.x, dy = v0.y - p.y, dz = v0.z - p.z;
     if (dx * dx + dy * dy + dz * dz > P.icp_dist2) return false;
-    if (!bilinear_taps(P.mnrm[l], W, u, v, x0, y0, rx, ry, n)) return false;
+    if (!bilerp(qn, rx, ry, n)) return false;
     n = normalised(n);
     A[0] = p.z * n.y + (-p.y) * n.z;
     A[1] = (-p.z) * n.x + p.x * n.z;

@@ -43,6 +43,24 @@ void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList
 void launch_integrate(const SessParams *P, BrickList bl, int tsdf_type, int weight_type, int grid, hipStream_t s);
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
                     hipStream_t s);
+// Several pyramid levels in one launch: segment k of the grid is level
+// level[k]'s work list (passed by value).
+struct LevelWork {
+    const WorkItem *w[kMaxLevels];
+    int n[kMaxLevels];
+    int level[kMaxLevels];
+    int nseg;
+    int blocks() const {
+        int t = 0;
+        for (int k = 0; k < nseg; ++k) t += n[k];
+        return t;
+    }
+};
+LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels);
+void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
+                          unsigned long long *stats, hipStream_t s);
+void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
+                          unsigned long long *stats, hipStream_t s);
 // volume maintenance (single session, P points at its parameters)
 void launch_volume_init(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, uint32_t nwords,
                         hipStream_t s);

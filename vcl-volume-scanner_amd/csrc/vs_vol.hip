@@ -453,12 +453,24 @@ __device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n
 // map with w = 1, a miss writes its final sentinels. k_raycast_shade then
 // evaluates the 32-tap gradient of every hit with all its loads in flight at
 // once and writes the final vertex / normal / shade (camera frame).
+// Work item and level of this workgroup in a multi-level launch.
+__device__ __forceinline__ WorkItem level_item(const LevelWork &lw, int &level) {
+    int b = blockIdx.x;
+    int s = 0;
+    while (s + 1 < lw.nseg && b >= lw.n[s]) {
+        b -= lw.n[s];
+        ++s;
+    }
+    level = lw.level[s];
+    return lw.w[s][b];
+}
+
 template <class TV, class TW, bool STATS>
-__global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__restrict__ SP,
-                                                          const WorkItem *__restrict__ work, int level,
+__global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__restrict__ SP, LevelWork lw,
                                                           unsigned long long *stats) {
     __shared__ uint32_t s_socc[kSoccLdsWords];
-    const WorkItem wi = work[blockIdx.x];
+    int level;
+    const WorkItem wi = level_item(lw, level);
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
     const LevelDesc &L = P.lev[level];
@@ -501,10 +513,10 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
 }
 
 template <class TV, class TW, bool STATS>
-__global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams *__restrict__ SP,
-                                                                const WorkItem *__restrict__ work, int level,
+__global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams *__restrict__ SP, LevelWork lw,
                                                                 unsigned long long *stats) {
-    const WorkItem wi = work[blockIdx.x];
+    int level;
+    const WorkItem wi = level_item(lw, level);
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
     const LevelDesc &L = P.lev[level];
@@ -631,25 +643,62 @@ void launch_integrate(const SessParams *P, BrickList bl, int tt, int wt, int gri
 }
 
 template <class TV, class TW>
-static void raycast_t(const SessParams *P, const WorkItem *work, int n, int level, unsigned long long *stats,
-                      hipStream_t s) {
+static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, hipStream_t s) {
+    const int n = lw.blocks();
     if (n <= 0) return;
-    if (stats) {
-        hipLaunchKernelGGL((k_raycast<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, work, level, stats);
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, work, level,
-                           stats);
-    } else {
-        hipLaunchKernelGGL((k_raycast<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, work, level, stats);
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, work, level,
-                           stats);
+    if (stats)
+        hipLaunchKernelGGL((k_raycast<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+    else
+        hipLaunchKernelGGL((k_raycast<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+}
+template <class TV, class TW>
+static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, hipStream_t s) {
+    const int n = lw.blocks();
+    if (n <= 0) return;
+    if (stats)
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+    else
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+}
+LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels) {
+    LevelWork lw{};
+    for (int l = levels - 1; l >= 0; --l)  // coarse levels first: their launches are the least parallel
+        if (n[l] > 0) {
+            lw.w[lw.nseg] = work[l];
+            lw.n[lw.nseg] = n[l];
+            lw.level[lw.nseg] = l;
+            ++lw.nseg;
+        }
+    return lw;
+}
+void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
+                          hipStream_t s) {
+    VS_DISPATCH(tt, wt, march_t, P, lw, stats, s);
+}
+void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
+                          hipStream_t s) {
+    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, s);
+}
+static LevelWork one_level(const WorkItem *work, int n, int level) {
+    LevelWork lw{};
+    if (n > 0) {
+        lw.w[0] = work;
+        lw.n[0] = n;
+        lw.level[0] = level;
+        lw.nseg = 1;
     }
+    return lw;
 }
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
-    VS_DISPATCH(tt, wt, raycast_t, P, work, n, level, nullptr, s);
+    const LevelWork lw = one_level(work, n, level);
+    launch_raycast_march(P, lw, tt, wt, nullptr, s);
+    launch_raycast_shade(P, lw, tt, wt, nullptr, s);
 }
 void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
                           unsigned long long *stats, hipStream_t s) {
-    VS_DISPATCH(tt, wt, raycast_t, P, work, n, level, stats, s);
+    const LevelWork lw = one_level(work, n, level);
+    launch_raycast_march(P, lw, tt, wt, stats, s);
+    launch_raycast_shade(P, lw, tt, wt, stats, s);
 }
 
 static int grid_for(size_t n) {
