@@ -209,7 +209,7 @@ __device__ __forceinline__ void block_terms(float v[kIcpVals], double *red64) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < kIcpVals; ++k) {
-        float s = wave_sum(v[k]);
+        float s = wave_sum_dpp(v[k]);
         if (lane == 0) wsum[wv][k] = s;
     }
     __syncthreads();
