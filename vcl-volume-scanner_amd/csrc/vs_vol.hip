@@ -165,23 +165,26 @@ struct Vol {
     const TV *F;
     const TW *Wt;
     const uint32_t *sbits;  // superbrick bits (LDS copy when it fits)
-    // occupancy of the last brick and superbrick looked up: a ray's
-    // consecutive taps mostly stay in one brick, and empty space is crossed a
-    // superbrick at a time; both bitmaps are read-only during the launch
-    mutable int cb = -1, csb = -1;
-    mutable bool cocc = false, csocc = false;
+    // occupancy of the last brick looked up (a ray's consecutive taps mostly
+    // stay in one brick; both bitmaps are read-only during the launch), in
+    // one register: brick index in bits 0-29 (< 2^24 bricks), bit 30 = its
+    // superbrick is occupied, bit 31 = the brick is occupied
+    static constexpr unsigned kIdx = 0x3FFFFFFFu, kSuperOcc = 1u << 30, kBrickOcc = 1u << 31;
+    mutable unsigned cbf = kIdx;
+    __device__ __forceinline__ int cb() const { return (int)(cbf & kIdx); }
+    __device__ __forceinline__ bool super_occupied() const { return cbf & kSuperOcc; }
     __device__ __forceinline__ bool occupied(int x, int y, int z) const {
-        const int b = brick_of(P, x, y, z);
-        if (b != cb) {
-            cb = b;
+        const unsigned b = (unsigned)brick_of(P, x, y, z);
+        if (b != (cbf & kIdx)) {
             const int sb = super_of_brick(P, x >> 3, y >> 3, z >> 3);
-            if (sb != csb) {
-                csb = sb;
-                csocc = (sbits[sb >> 5] >> (sb & 31)) & 1u;
+            unsigned f = b;
+            if ((sbits[sb >> 5] >> (sb & 31)) & 1u) {
+                f |= kSuperOcc;
+                if ((P.bocc[b >> 5] >> (b & 31)) & 1u) f |= kBrickOcc;
             }
-            cocc = csocc && ((P.bocc[b >> 5] >> (b & 31)) & 1u);
+            cbf = f;
         }
-        return cocc;
+        return cbf & kBrickOcc;
     }
     __device__ __forceinline__ bool in_grid(int x, int y, int z) const {
         return x >= 0 && x < P.vdim[0] && y >= 0 && y < P.vdim[1] && z >= 0 && z < P.vdim[2];
@@ -200,7 +203,7 @@ struct Vol {
         out = 1.0f;
         if (!in_grid(x, y, z)) return false;
         if (!occupied(x, y, z)) return false;
-        return load((size_t)cb * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out);
+        return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out);
     }
     // d_gtvu: nearest voxel by ROUNDF. Returns 1 (data), 0 (no data) or -1
     // when the rounded position lies outside the grid on an axis along which
@@ -224,12 +227,12 @@ struct Vol {
             return 0;
         }
         if (!occupied(x, y, z)) {
-            const int sh = csocc ? 3 : 5;  // empty brick, or a whole empty superbrick
+            const int sh = super_occupied() ? 3 : 5;  // empty brick, or a whole empty superbrick
             lo = make_int3((x >> sh) << sh, (y >> sh) << sh, (z >> sh) << sh);
             size = 1 << sh;
             return 0;
         }
-        return load((size_t)cb * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
+        return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
     }
     __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out) const {
         int3 lo;
@@ -326,7 +329,32 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
     bool behind = false;
     float total = 0.0f;
     int iter = 0;
+    // Brent cycle detection on the fine-search state (position, tsdf): the
+    // next state is a function of it alone (the volume does not change), and
+    // neither a hit nor the surface-behind exit lies on a cycle, so a state
+    // seen before means the ray can only end as a miss at the iteration or
+    // distance cap. Fine-mode rays that bounce across a voxel of F = -1.0
+    // (common with FP16 values) do this; ending them at once saves up to
+    // max_iter iterations each. Coarse steps only move forward, and the mode
+    // never returns to coarse, so coarse states need no check.
+    // The saved state is refreshed after 1, 2, 4, ... steps; the step count
+    // since the last save and log2 of the interval share one register (a
+    // match is a genuine repeat whatever the schedule).
+    unsigned sv_x = 0xFFFFFFFFu, sv_y = 0xFFFFFFFFu, sv_z = 0xFFFFFFFFu, sv_t = 0xFFFFFFFFu, brent = 0;
     while (true) {
+        if (!STATS && !coarse) {
+            const unsigned bx = __float_as_uint(vox.x), by = __float_as_uint(vox.y), bz = __float_as_uint(vox.z),
+                           bt = __float_as_uint(tsdf);
+            if (bx == sv_x && by == sv_y && bz == sv_z && bt == sv_t) return false;
+            if ((brent & 0xFFFFu) == (1u << (brent >> 16))) {
+                sv_x = bx;
+                sv_y = by;
+                sv_z = bz;
+                sv_t = bt;
+                brent = ((brent >> 16) + 1) << 16;
+            }
+            ++brent;
+        }
         if (++iter > C.max_iter) return false;
         const float step = coarse ? C.step_scale
                                   : ((tsdf > 0.0f) ? fmaxf(C.step_scale * tsdf, 1.0f) : fminf(C.step_scale * tsdf, -1.0f));
