@@ -40,59 +40,70 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
 // Neighbouring pixels hit the same few bricks, so a workgroup first
 // deduplicates its up to 6 x 256 samples in an LDS hash set; only the first
 // insertion of a brick touches the device-wide bitmap (device-scope atomics are
-// performed past the per-XCD L2 and are the expensive part).
-constexpr int kVisSet = 2048;  // > 6 samples x 256 pixels, so probing always ends
-__device__ __forceinline__ void mark_brick(const SessParams &P, int sess, uint32_t b, BrickList bl) {
-    const uint32_t bit = 1u << (b & 31u);
-    uint32_t *word = P.bvis + (b >> 5);
-    if (__atomic_load_n(word, __ATOMIC_RELAXED) & bit) return;
-    const uint32_t old = atomicOr(word, bit);
-    if (!(old & bit)) {
-        const unsigned idx = atomicAdd(bl.count, 1u);
-        if (idx < bl.capacity) bl.items[idx] = make_int2(sess, (int)b);
-        atomicAdd(&P.st->vis_count, 1u);
-    }
-}
+// performed past the per-XCD L2 and are the expensive part). The bricks this
+// workgroup marks first are gathered in LDS and appended with one reservation
+// on the list counter (one same-address atomic per workgroup, not per brick).
+constexpr int kVisSet = 2048;   // > 6 samples x 256 pixels, so probing always ends
+constexpr int kVisNew = 6 * kTileThreads;
 
 __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__restrict__ SP,
                                                            const WorkItem *__restrict__ work, BrickList bl) {
     __shared__ int set[kVisSet];
+    __shared__ int fresh[kVisNew];
+    __shared__ unsigned nfresh, base;
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
-    const SessState *st = P.st;
+    SessState *st = P.st;
     const LevelDesc &L = P.lev[0];
     if (!st->fuse_ok || tile_out(L, wi.tile)) return;
     for (int i = threadIdx.x; i < kVisSet; i += kTileThreads) set[i] = -1;
+    if (threadIdx.x == 0) nfresh = 0u;
     __syncthreads();
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
-    if (x >= L.W || y >= L.H) return;
-    const float td = depth_m(P.raw[(size_t)y * L.W + x]);
-    if (!(td > 0.0f)) return;
-    const float *Tcg = st->T_cg_f;
-    const f3 p = transform(Tcg, back_project((float)x, (float)y, L.K, td));
-    const f3 pe = transform(Tcg, back_project((float)x, (float)y, L.K, 100.0f));
-    const f3 dir = normalised(sub(pe, p));
-    for (int n = -4; n <= 1; ++n) {
-        const f3 q = add(p, scale((float)n * P.mu, dir));
-        f3 tv;
-        if (!world_to_voxel(P, q, tv)) continue;
-        // ROUNDF(v)/8 with the macro's precedence: the negative branch is not divided
-        const int bx = (tv.x < 0.0f) ? (int)(tv.x - 0.5f) : (int)(tv.x + 0.5f) / kBrick;
-        const int by = (tv.y < 0.0f) ? (int)(tv.y - 0.5f) : (int)(tv.y + 0.5f) / kBrick;
-        const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)(tv.z + 0.5f) / kBrick;
-        if (bx < 0 || bx >= P.bdim[0] || by < 0 || by >= P.bdim[1] || bz < 0 || bz >= P.bdim[2]) break;
-        const uint32_t b = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;
-        uint32_t h = (b * 2654435761u) >> 21;  // 11-bit multiplicative hash
-        while (true) {
-            const int old = atomicCAS(&set[h], -1, (int)b);
-            if (old == -1) {
-                mark_brick(P, wi.sess, b, bl);
-                break;
+    const float td = (x < L.W && y < L.H) ? depth_m(P.raw[(size_t)y * L.W + x]) : 0.0f;
+    if (td > 0.0f) {
+        const float *Tcg = st->T_cg_f;
+        const f3 p = transform(Tcg, back_project((float)x, (float)y, L.K, td));
+        const f3 pe = transform(Tcg, back_project((float)x, (float)y, L.K, 100.0f));
+        const f3 dir = normalised(sub(pe, p));
+        for (int n = -4; n <= 1; ++n) {
+            const f3 q = add(p, scale((float)n * P.mu, dir));
+            f3 tv;
+            if (!world_to_voxel(P, q, tv)) continue;
+            // ROUNDF(v)/8 with the macro's precedence: the negative branch is not divided
+            const int bx = (tv.x < 0.0f) ? (int)(tv.x - 0.5f) : (int)(tv.x + 0.5f) / kBrick;
+            const int by = (tv.y < 0.0f) ? (int)(tv.y - 0.5f) : (int)(tv.y + 0.5f) / kBrick;
+            const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)(tv.z + 0.5f) / kBrick;
+            if (bx < 0 || bx >= P.bdim[0] || by < 0 || by >= P.bdim[1] || bz < 0 || bz >= P.bdim[2]) break;
+            const uint32_t b = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;
+            uint32_t h = (b * 2654435761u) >> 21;  // 11-bit multiplicative hash
+            while (true) {
+                const int old = atomicCAS(&set[h], -1, (int)b);
+                if (old == -1) {
+                    // first sample of this brick in the workgroup: set its bit
+                    const uint32_t bit = 1u << (b & 31u);
+                    uint32_t *word = P.bvis + (b >> 5);
+                    if (!(__atomic_load_n(word, __ATOMIC_RELAXED) & bit) && !(atomicOr(word, bit) & bit))
+                        fresh[atomicAdd(&nfresh, 1u)] = (int)b;
+                    break;
+                }
+                if (old == (int)b) break;
+                h = (h + 1) & (kVisSet - 1);
             }
-            if (old == (int)b) break;
-            h = (h + 1) & (kVisSet - 1);
         }
+    }
+    __syncthreads();
+    const unsigned nf = nfresh;
+    if (nf == 0u) return;
+    if (threadIdx.x == 0) {
+        base = atomicAdd(bl.count, nf);
+        atomicAdd(&st->vis_count, nf);
+    }
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < nf; i += kTileThreads) {
+        const unsigned idx = base + i;
+        if (idx < bl.capacity) bl.items[idx] = make_int2(wi.sess, fresh[i]);
     }
 }
 
@@ -107,6 +118,9 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
     const unsigned n = min(*bl.count, bl.capacity);
     const int t = threadIdx.x;
     const int xl = t & 7, yl = (t >> 3) & 7, zl = t >> 6;
+    // updated-voxel count of the current run of same-session bricks (thread 0)
+    int run_sess = -1;
+    unsigned long long run_upd = 0ull;
     for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
         const int2 it = bl.items[i];
         const SessParams &P = SP[it.x];
@@ -144,12 +158,22 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
         }
         const int c1 = __syncthreads_count(nupd >= 1), c2 = __syncthreads_count(nupd >= 2);
         if (t == 0 && c1 > 0) {
-            atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
+            // occupancy bits only ever get set: skip the device atomic when a
+            // plain read already shows the bit (a stale 0 costs one atomic)
+            const uint32_t bb = 1u << (b & 31);
+            if (!(__atomic_load_n(P.bocc + (b >> 5), __ATOMIC_RELAXED) & bb)) atomicOr(P.bocc + (b >> 5), bb);
             const int sb = super_of_brick(P, bx, by, bz);
-            atomicOr(P.socc + (sb >> 5), 1u << (sb & 31));
-            atomicAdd(&P.st->upd_count, (unsigned long long)(c1 + c2));
+            const uint32_t sbb = 1u << (sb & 31);
+            if (!(__atomic_load_n(P.socc + (sb >> 5), __ATOMIC_RELAXED) & sbb)) atomicOr(P.socc + (sb >> 5), sbb);
+            if (it.x != run_sess) {
+                if (run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
+                run_sess = it.x;
+                run_upd = 0ull;
+            }
+            run_upd += (unsigned long long)(c1 + c2);
         }
     }
+    if (t == 0 && run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
 }
 
 // ---------------------------------------------------------------------------
