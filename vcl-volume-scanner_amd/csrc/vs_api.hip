@@ -53,8 +53,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, blist, bcount, params, work[kMaxLevels],
-        stats;
+    size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, lists, params, stats;
     size_t total;
     size_t nvox;
     uint32_t nbricks, nwords, nsuper, nswords;
@@ -63,6 +62,19 @@ struct Layout {
 };
 
 int tiles_of(int W, int H) { return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH); }
+
+// Bytes of a batch's list region: per level the (slot, tile) items of every
+// tile and the (slot, group) items of every ICP tile group, then the brick
+// list and its counter.
+size_t lists_bytes(const size_t ntile[kMaxLevels], const size_t ngroup[kMaxLevels], size_t bricks) {
+    size_t off = 0;
+    for (int l = 0; l < kMaxLevels; ++l) {
+        off = align_up(off + std::max<size_t>(ntile[l], 1) * sizeof(WorkItem));
+        off = align_up(off + std::max<size_t>(ngroup[l], 1) * sizeof(WorkItem));
+    }
+    off = align_up(off + std::max<size_t>(bricks, 1) * sizeof(int2));
+    return align_up(off + 16);
+}
 
 Layout make_layout(const vs_session_desc &d) {
     Layout L{};
@@ -91,7 +103,6 @@ Layout make_layout(const vs_session_desc &d) {
         L.mvtx[l] = take(n * 16);
         L.mnrm[l] = take(n * 16);
         L.mimg[l] = take(n);
-        L.work[l] = take((size_t)std::max(L.tiles[l], 1) * sizeof(WorkItem));
     }
     L.tsdf = take(L.nvox * sv);
     L.weight = take(L.nvox * sw);
@@ -101,8 +112,14 @@ Layout make_layout(const vs_session_desc &d) {
     L.icp_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_mask = take((size_t)W * H);
-    L.blist = take((size_t)L.nbricks * sizeof(int2));
-    L.bcount = take(16);
+    {
+        size_t nt[kMaxLevels], ng[kMaxLevels];
+        for (int l = 0; l < kMaxLevels; ++l) {
+            nt[l] = (size_t)L.tiles[l];
+            ng[l] = (size_t)icp_groups(L.tiles[l]);
+        }
+        L.lists = take(lists_bytes(nt, ng, L.nbricks));
+    }
     L.stats = take(64);
     L.total = off;
     return L;
@@ -137,8 +154,10 @@ struct Batch {
     int n = 0, tt = 0, wt = 0, levels = 0;
     int max_iters[kMaxLevels] = {0, 0, 0};
     SessParams *dP = nullptr;  // device array
-    WorkItem *dWork[kMaxLevels] = {nullptr, nullptr, nullptr};
+    WorkItem *dWork[kMaxLevels] = {nullptr, nullptr, nullptr};   // (slot, tile)
     int nWork[kMaxLevels] = {0, 0, 0};
+    WorkItem *dGroup[kMaxLevels] = {nullptr, nullptr, nullptr};  // (slot, ICP tile group)
+    int nGroup[kMaxLevels] = {0, 0, 0};
     BrickList bl{};
     int integ_grid = 1;
     void *owned = nullptr;  // allocation owned by a context batch (null for a session's own)
@@ -221,6 +240,57 @@ struct vs_server {
 
 namespace {
 
+// Host copy of a batch's work lists (see lists_bytes for the device layout).
+struct ListSet {
+    std::vector<WorkItem> tile[kMaxLevels], group[kMaxLevels];
+    uint32_t bricks = 0;
+    int levels = 0, max_iters[kMaxLevels] = {0, 0, 0};
+    void add(int slot, const vs_session *s) {
+        levels = std::max(levels, s->d.levels);
+        for (int l = 0; l < kMaxLevels; ++l) {
+            max_iters[l] = std::max(max_iters[l], s->hp.iters[l]);
+            for (int t = 0; t < s->L.tiles[l]; ++t) tile[l].push_back({slot, t});
+            for (int g = 0; g < icp_groups(s->L.tiles[l]); ++g) group[l].push_back({slot, g});
+        }
+        bricks += s->L.nbricks;
+    }
+    size_t bytes() const {
+        size_t nt[kMaxLevels], ng[kMaxLevels];
+        for (int l = 0; l < kMaxLevels; ++l) {
+            nt[l] = tile[l].size();
+            ng[l] = group[l].size();
+        }
+        return lists_bytes(nt, ng, bricks);
+    }
+    // Points the batch at `mem` (bytes() long), uploads the lists and clears
+    // the brick counter; sets the iteration bounds and the integrate grid.
+    vs_status place(Batch &B, char *mem, int integ_max) const {
+        size_t off = 0;
+        for (int l = 0; l < kMaxLevels; ++l) {
+            B.dWork[l] = (WorkItem *)(mem + off);
+            B.nWork[l] = (int)tile[l].size();
+            off = align_up(off + std::max<size_t>(tile[l].size(), 1) * sizeof(WorkItem));
+            B.dGroup[l] = (WorkItem *)(mem + off);
+            B.nGroup[l] = (int)group[l].size();
+            off = align_up(off + std::max<size_t>(group[l].size(), 1) * sizeof(WorkItem));
+            if (!tile[l].empty())
+                HIP_TRY(hipMemcpy(B.dWork[l], tile[l].data(), tile[l].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+            if (!group[l].empty())
+                HIP_TRY(hipMemcpy(B.dGroup[l], group[l].data(), group[l].size() * sizeof(WorkItem),
+                                  hipMemcpyHostToDevice));
+        }
+        B.bl.items = (int2 *)(mem + off);
+        off = align_up(off + std::max<size_t>(bricks, 1) * sizeof(int2));
+        B.bl.count = (unsigned *)(mem + off);
+        B.bl.capacity = bricks;
+        HIP_TRY(hipMemset(B.bl.count, 0, 16));
+        B.levels = levels;
+        for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = max_iters[l];
+        B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(bricks, (uint32_t)integ_max));
+        return VS_OK;
+    }
+};
+
 void free_batch_graph(Batch &b) {
     if (b.gexec) hipGraphExecDestroy(b.gexec);
     if (b.graph) hipGraphDestroy(b.graph);
@@ -281,49 +351,21 @@ vs_status server_forget(vs_session *s) {
 vs_status server_rebuild(vs_server *v) {
     Batch &B = v->B;
     free_batch_graph(B);
-    std::vector<WorkItem> wk[kMaxLevels];
-    uint32_t cap = 0;
-    B.levels = 0;
-    for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = 0;
-    for (int i = 0; i < (int)v->slot.size(); ++i) {
-        const vs_session *s = v->slot[i];
-        if (!s) continue;
-        B.levels = std::max(B.levels, s->d.levels);
-        for (int l = 0; l < kMaxLevels; ++l) {
-            B.max_iters[l] = std::max(B.max_iters[l], s->hp.iters[l]);
-            for (int t = 0; t < s->L.tiles[l]; ++t) wk[l].push_back({i, t});
-        }
-        cap += s->L.nbricks;
-    }
-    size_t off = 0, offW[kMaxLevels];
-    for (int l = 0; l < kMaxLevels; ++l) {
-        offW[l] = off;
-        off = align_up(off + std::max<size_t>(wk[l].size(), 1) * sizeof(WorkItem));
-    }
-    const size_t offB = off;
-    off = align_up(off + (size_t)std::max<uint32_t>(cap, 1) * sizeof(int2));
-    const size_t offC = off;
-    off = align_up(off + 16);
-    if (off > v->lists_bytes) {
+    ListSet ls;
+    for (int i = 0; i < (int)v->slot.size(); ++i)
+        if (v->slot[i]) ls.add(i, v->slot[i]);
+    const size_t need = ls.bytes();
+    if (need > v->lists_bytes) {
         if (v->lists) HIP_TRY(hipFree(v->lists));
         v->lists = nullptr;
         v->lists_bytes = 0;
-        const size_t want = off + off / 2;
+        const size_t want = need + need / 2;
         hipError_t e = hipMalloc((void **)&v->lists, want);
         if (e != hipSuccess) return fail(VS_ENOMEM, "server lists (%zu bytes): %s", want, hipGetErrorString(e));
         v->lists_bytes = want;
     }
-    for (int l = 0; l < kMaxLevels; ++l) {
-        B.dWork[l] = (WorkItem *)(v->lists + offW[l]);
-        B.nWork[l] = (int)wk[l].size();
-        if (!wk[l].empty())
-            HIP_TRY(hipMemcpy(B.dWork[l], wk[l].data(), wk[l].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-    }
-    B.bl.items = (int2 *)(v->lists + offB);
-    B.bl.count = (unsigned *)(v->lists + offC);
-    B.bl.capacity = cap;
-    HIP_TRY(hipMemset(B.bl.count, 0, 16));
-    B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(cap, 8192));
+    vs_status r = ls.place(B, v->lists, 8192);
+    if (r != VS_OK) return r;
     v->dirty = false;
     return VS_OK;
 }
@@ -349,7 +391,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     for (int l = 1; l < B.levels; ++l) launch_pyr_down(B.dP, B.dWork[l], B.nWork[l], l, s);
     mark(2);
     for (int l = B.levels - 1; l >= 0; --l)
-        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], B.n, l, it, s);
+        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dGroup[l], B.nGroup[l], B.n, l, it, s);
     launch_track_finalize(B.dP, B.n, s);
     mark(3);
     launch_vis_clear(B.dP, B.n, B.bl, s);
@@ -573,26 +615,17 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     B.n = 1;
     B.tt = d.tsdf_type;
     B.wt = d.weight_type;
-    B.levels = d.levels;
-    for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = P.iters[l];
     B.dP = (SessParams *)(m + L.params);
-    std::vector<WorkItem> wk;
-    for (int l = 0; l < kMaxLevels; ++l) {
-        B.dWork[l] = (WorkItem *)(m + L.work[l]);
-        B.nWork[l] = L.tiles[l];
-        wk.resize(std::max(L.tiles[l], 1));
-        for (int t = 0; t < L.tiles[l]; ++t) wk[t] = {0, t};
-        HIP_TRY(hipMemcpy(B.dWork[l], wk.data(), wk.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    {
+        ListSet ls;
+        ls.add(0, s.get());
+        vs_status r = ls.place(B, m + L.lists, 2048);
+        if (r != VS_OK) return r;
     }
-    B.bl.items = (int2 *)(m + L.blist);
-    B.bl.count = (unsigned *)(m + L.bcount);
-    B.bl.capacity = L.nbricks;
-    B.integ_grid = (int)std::min<uint32_t>(L.nbricks, 2048);
     B.members = {s.get()};
     HIP_TRY(hipMemcpy(B.dP, &P, sizeof(P), hipMemcpyHostToDevice));
     // state: pose, counters; volume F = 1, W = 0
     HIP_TRY(hipMemset(m + L.state, 0, sizeof(SessState)));
-    HIP_TRY(hipMemset(m + L.bcount, 0, 16));
     for (int l = 0; l < kMaxLevels; ++l) {
         const size_t n = (size_t)(d.width >> l) * (d.height >> l);
         HIP_TRY(hipMemset(m + L.filt[l], 0, n * 4));
@@ -693,7 +726,7 @@ vs_status vs_track(vs_session *s, const double *T_01_init, vs_stream stream) {
     const Batch &B = s->self;
     launch_track_setup_forced(B.dP, 1, T_01_init, h);
     for (int l = B.levels - 1; l >= 0; --l)
-        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dWork[l], B.nWork[l], B.n, l, it, h);
+        for (int it = 0; it < B.max_iters[l]; ++it) launch_icp_iter(B.dP, B.dGroup[l], B.nGroup[l], B.n, l, it, h);
     launch_track_finalize(B.dP, 1, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
@@ -752,42 +785,20 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
         nb->wt = key[0]->d.weight_type;
         nb->members = key;
         std::vector<SessParams> hp(n);
-        std::vector<WorkItem> wk[kMaxLevels];
-        uint32_t cap = 0;
+        ListSet ls;
         for (int i = 0; i < n; ++i) {
             hp[i] = key[i]->hp;
-            nb->levels = std::max(nb->levels, key[i]->d.levels);
-            for (int l = 0; l < kMaxLevels; ++l) {
-                nb->max_iters[l] = std::max(nb->max_iters[l], key[i]->hp.iters[l]);
-                for (int t = 0; t < key[i]->L.tiles[l]; ++t) wk[l].push_back({i, t});
-            }
-            cap += key[i]->L.nbricks;
+            ls.add(i, key[i]);
         }
-        size_t off = 0, offP, offW[kMaxLevels], offB, offC;
-        offP = off; off = align_up(off + n * sizeof(SessParams));
-        for (int l = 0; l < kMaxLevels; ++l) {
-            offW[l] = off;
-            off = align_up(off + std::max<size_t>(wk[l].size(), 1) * sizeof(WorkItem));
-        }
-        offB = off; off = align_up(off + (size_t)cap * sizeof(int2));
-        offC = off; off = align_up(off + 16);
+        const size_t offP = 0, offL = align_up(n * sizeof(SessParams)), total = offL + ls.bytes();
         char *mem = nullptr;
-        hipError_t e = hipMalloc((void **)&mem, off);
-        if (e != hipSuccess) return fail(VS_ENOMEM, "batch allocation (%zu bytes): %s", off, hipGetErrorString(e));
+        hipError_t e = hipMalloc((void **)&mem, total);
+        if (e != hipSuccess) return fail(VS_ENOMEM, "batch allocation (%zu bytes): %s", total, hipGetErrorString(e));
         nb->owned = mem;
         nb->dP = (SessParams *)(mem + offP);
         HIP_TRY(hipMemcpy(nb->dP, hp.data(), n * sizeof(SessParams), hipMemcpyHostToDevice));
-        for (int l = 0; l < kMaxLevels; ++l) {
-            nb->dWork[l] = (WorkItem *)(mem + offW[l]);
-            nb->nWork[l] = (int)wk[l].size();
-            if (!wk[l].empty())
-                HIP_TRY(hipMemcpy(nb->dWork[l], wk[l].data(), wk[l].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-        }
-        nb->bl.items = (int2 *)(mem + offB);
-        nb->bl.count = (unsigned *)(mem + offC);
-        nb->bl.capacity = cap;
-        HIP_TRY(hipMemset(nb->bl.count, 0, 16));
-        nb->integ_grid = (int)std::min<uint32_t>(cap, 8192);
+        vs_status r = ls.place(*nb, mem + offL, 8192);
+        if (r != VS_OK) return r;
         B = nb.get();
         ctx->batches[key] = std::move(nb);
     } else {

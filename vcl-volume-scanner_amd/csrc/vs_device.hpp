@@ -21,7 +21,11 @@ constexpr int kBrickVox = 512;
 constexpr int kTileW = 32;          // image tile of one 256-thread workgroup: 32 x 8 pixels
 constexpr int kTileH = 8;
 constexpr int kTileThreads = kTileW * kTileH;
-constexpr int kIcpVals = 29;        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
+constexpr int kIcpVals = 29;
+// ICP tiles summed by one workgroup (a fixed grouping of each session's
+// tiles, so the sums do not depend on the batch)
+constexpr int kIcpGroup = 4;
+__host__ __device__ constexpr int icp_groups(int tiles) { return (tiles + kIcpGroup - 1) / kIcpGroup; }        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
 constexpr int kSuper = 32;          // superbrick edge (voxels) = 4 bricks
 constexpr int kSoccLdsWords = 1024; // superbrick bits a raycast workgroup keeps in LDS (up to 1024^3 voxels)
 

@@ -4,12 +4,13 @@
 // Reference: LGKvPoseTracker::tp_Track_Pose (z_yooji_cuda_pose_tracker.cu:790-1042)
 // launches one summand+reduction kernel per iteration, copies ~200 KB of
 // block partials to the host, sums them in FP32, solves on the host and copies
-// the pose back. Here each iteration is one launch whose last workgroup (per
-// session) sums the partials in FP64 in tile order, solves the 6x6 system
-// (Cholesky, FP64), applies the SE(3) update and publishes the FP32 pose the
-// next launch reads. Convergence (|x|_inf < 5e-7) and failure are flags the
-// following launches test, so the host enqueues a fixed launch sequence that
-// can be captured in a graph.
+// the pose back. Here each iteration is two launches: k_icp_iter writes one
+// FP64 partial of the 29 sums per group of tiles, then k_icp_solve (one
+// workgroup per session) sums them in FP64 in group order, solves the 6x6
+// system (Cholesky, FP64), applies the SE(3) update and publishes the FP32
+// pose the next iteration reads. Convergence (|x|_inf < 5e-7) and failure are
+// flags the following launches test, so the host enqueues a fixed launch
+// sequence that can be captured in a graph.
 #include "vs_device.hpp"
 #include "vs_launch.hpp"
 
@@ -227,10 +228,13 @@ __device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *s
 }
 
 // One Gauss-Newton iteration at `level`, first half: the 29 sums of every
-// 32x8 tile of every session of the work list, one FP64 partial per tile.
-// No cross-workgroup synchronisation here: on a multi-XCD device an
-// agent-scope release per workgroup would write back L2 each time, so the
-// reduction and the solve are the next launch (k_icp_solve).
+// group of kIcpGroup consecutive 32x8 tiles of every session of the work
+// list (items are (session, group)), one FP64 partial per group. Each thread
+// accumulates its pixel of each tile of the group in FP32, then one block
+// reduction runs per group. No cross-workgroup synchronisation here: on a
+// multi-XCD device an agent-scope release per workgroup would write back L2
+// each time, so the reduction and the solve are the next launch
+// (k_icp_solve).
 __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__restrict__ SP,
                                                            const WorkItem *__restrict__ work, int level, int iter) {
     __shared__ float T21s[16];
@@ -240,22 +244,32 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     const SessState *st = P.st;
     if (level >= P.levels || icp_skip(P, st, level, iter)) return;
     const LevelDesc &L = P.lev[level];
-    if (tile_out(L, wi.tile)) return;
+    const int ntiles = L.tiles_x * L.tiles_y;
+    const int t0 = wi.tile * kIcpGroup;
+    if (t0 >= ntiles) return;
     if (threadIdx.x < 16) T21s[threadIdx.x] = st->T_21_f[threadIdx.x];
     __syncthreads();
-    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
-    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
-    float A[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
-    bool ok = false;
-    if (x < L.W && y < L.H) ok = icp_summand(P, level, x, y, T21s, A, b);
-    float v[kIcpVals];
-    pixel_terms(ok, A, b, v);
-    block_terms(v, red64);
+    float acc[kIcpVals];
+#pragma unroll
+    for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0f;
+    for (int g = 0; g < kIcpGroup && t0 + g < ntiles; ++g) {
+        const int tile = t0 + g;
+        const int x = (tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+        const int y = (tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+        float A[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
+        bool ok = false;
+        if (x < L.W && y < L.H) ok = icp_summand(P, level, x, y, T21s, A, b);
+        float v[kIcpVals];
+        pixel_terms(ok, A, b, v);
+#pragma unroll
+        for (int k = 0; k < kIcpVals; ++k) acc[k] += v[k];
+    }
+    block_terms(acc, red64);
     if (threadIdx.x < kIcpVals) P.icp_part[(size_t)wi.tile * kIcpVals + threadIdx.x] = red64[threadIdx.x];
 }
 
-// Second half: one workgroup per session sums the tile partials in FP64 in
-// tile order (so the result does not depend on the batch or the schedule),
+// Second half: one workgroup per session sums the group partials in FP64 in
+// group order (so the result does not depend on the batch or the schedule),
 // solves the 6x6 system and applies the SE(3) update.
 __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__restrict__ SP, int nsess, int level,
                                                             int iter) {
@@ -265,11 +279,11 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
     SessState *st = P.st;
     if (level >= P.levels || icp_skip(P, st, level, iter)) return;
     const LevelDesc &L = P.lev[level];
-    const int ntiles = L.tiles_x * L.tiles_y;
+    const int ngroups = icp_groups(L.tiles_x * L.tiles_y);
     double acc[kIcpVals];
 #pragma unroll
     for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0;
-    for (int t = threadIdx.x; t < ntiles; t += kTileThreads) {
+    for (int t = threadIdx.x; t < ngroups; t += kTileThreads) {
         const double *pp = P.icp_part + (size_t)t * kIcpVals;
 #pragma unroll
         for (int k = 0; k < kIcpVals; ++k) acc[k] += pp[k];
