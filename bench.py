@@ -131,7 +131,7 @@ def cpu_baseline(frames):
 
 
 def serve(ctx, streams, bufs, frames, W, H, seconds, with_plain):
-    """Serving form: Poisson session arrivals (2/s after 16 at start),
+    """Serving form: Poisson session arrivals (2/s after 64 at start),
     lifetimes 150-900 frames at 30 fps +-5 ms, volumes drawn from
     {128,256,384,512}^3 FP16 + u8, admitted under a 200 GB budget into a
     128-slot table. Frames stream from the resident pools (forward/backward
@@ -141,7 +141,7 @@ def serve(ctx, streams, bufs, frames, W, H, seconds, with_plain):
 
     w = vscan.WORKLOADS["fp16_640x480_512"]
     K = streams[0].K
-    reqs = serving.make_requests(7, seconds, len(bufs))
+    reqs = serving.make_requests(7, seconds, len(bufs), initial=64)
     ptr = lambda scene, k: bufs[scene].ptr.value + k * W * H * 2  # noqa: E731
     pose0 = lambda scene: streams[scene].pose(0)  # noqa: E731
 
@@ -158,12 +158,12 @@ def serve(ctx, streams, bufs, frames, W, H, seconds, with_plain):
     res = loop.run(reqs, seconds, log=log)
     loop.close()
     res["definition"] = ("latency = completion of the step that fused the frame (GPU event) - frame arrival; "
-                         "sessions arrive Poisson 2/s after 16 at t=0, 30 fps +-5 ms, 150-900 frames, volumes "
+                         "sessions arrive Poisson 2/s after 64 at t=0, 30 fps +-5 ms, 150-900 frames, volumes "
                          "{128,256,384,512}^3 FP16+u8, 200 GB budget, 128 slots")
     log(f"  batched: {res['fused_frames_per_s']} fused frames/s, p50 {res['latency_ms']['p50']} ms, "
         f"p99 {res['latency_ms']['p99']} ms")
     if with_plain:
-        reqs = serving.make_requests(7, seconds, len(bufs))
+        reqs = serving.make_requests(7, seconds, len(bufs), initial=64)
         log("serving: same trace, plain serial pipeline")
         loop = serving.ServeLoop(serving.SerialBackend(ctx, 128, 200 << 30), desc32, ptr, frames, pose0,
                                  max_batch=1)
