@@ -106,6 +106,19 @@ def raycast_bytes_level(sessions, level, sv_sw):
     return tot
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_batch64_640x480_512_fp16.json")
+
+
+def measured_traffic(kernel, args):
+    """Memory bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known
+    size); None for other workloads or when no such measurement exists."""
+    if args.sessions != 64 or args.volume != 512 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f).get("traffic_bytes", {}).get(kernel)
+
+
 def cpu_baseline(frames):
     """The reference algorithm's CPU path (the oracle instantiated in FP32 with
     OpenMP) on one session of the same kind: 640x480, 512^3 volume held in FP32
@@ -188,6 +201,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the serving-form run (0 = skip)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="end with one device-to-device copy of this size (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-plain", type=int, default=1,
                     help="also run the same request trace through the plain serial pipeline")
     args = ap.parse_args()
@@ -266,8 +281,9 @@ def main():
         b, ms = cand[name]
         ach = b / (ms * 1e-3) / 1e9
         out["roofline"] = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                           "algorithmic_bytes_per_launch": int(b), "avg_launch_ms": round(float(ms), 4)}
+                           "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                           "traffic": measured_traffic(name, args), "algorithmic_bytes_per_launch": int(b),
+                           "avg_launch_ms": round(float(ms), 4)}
         out["other_kernels"] = {k: {"GB/s": round(v[0] / (v[1] * 1e-3) / 1e9, 1), "ms": round(float(v[1]), 4),
                                     "bytes": int(v[0])} for k, v in cand.items()}
         upd = sum(s.integrate_counts()[1] for s in sessions)
@@ -293,6 +309,14 @@ def main():
     for s in sessions:
         s.close()
     for b in bufs:
+        b.free()
+    if args.calib_copy_mib > 0:
+        nb = args.calib_copy_mib << 20
+        a, b = hiprt.DeviceBuffer(nb), hiprt.DeviceBuffer(nb)
+        hiprt.copy_d2d(b, a, nb)
+        hiprt.device_sync()
+        log(f"calibration copy: {nb} bytes device to device")
+        a.free()
         b.free()
     ctx.close()
     print(json.dumps(out))

@@ -110,6 +110,12 @@ class Event:
             self.h = C.c_void_p()
 
 
+def copy_d2d(dst, src, nbytes):
+    """Device-to-device copy between two DeviceBuffers (synchronous)."""
+    assert nbytes <= dst.nbytes and nbytes <= src.nbytes
+    _ok(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
 def device_sync():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
