@@ -55,6 +55,15 @@ typedef enum {
 
 typedef enum { VS_TSDF_F32 = 0, VS_TSDF_F16 = 1 } vs_tsdf_type;
 typedef enum { VS_WEIGHT_F32 = 0, VS_WEIGHT_U8 = 1 } vs_weight_type;
+/* Integration operator:
+ *   VS_INTEGRATE_DEPTH: g_ctv_Compute_TSDF_of_Voxels (z_yooji_cuda_volume_integrator.cu:154-262),
+ *     the depth operator of cdtoc (:936-972);
+ *   VS_INTEGRATE_COLOR: the production g_ctvc_Compute_TSDF_of_Voxels_with_Color (:265-440) as
+ *     cdtocc (:976-1031) drives it from so_Scan_Object (z_yooji_object_scanner.cpp:1274-1281)
+ *     with a colour frame registered to the depth frame: SDF along the pixel ray,
+ *     voxels at the weight cap frozen, an RGB running average per voxel, and a
+ *     rendered texture map per pyramid level (tsdf_renderer.cu:1166-1184). */
+typedef enum { VS_INTEGRATE_DEPTH = 0, VS_INTEGRATE_COLOR = 1 } vs_integrate_mode;
 
 /* One scan session: camera, pyramid, tracker, volume and filter settings.
  * Replaces CKvYooji_Scanner_Parameter (STAC/Object/CPU/_yooji_scanner_parameter.h:4-71)
@@ -81,6 +90,7 @@ typedef struct {
     float bil_sigma_s;           /* bilateral spatial sigma (pixels) */
     float bil_sigma_r;           /* bilateral range sigma (m) */
     int bil_radius;              /* bilateral half window (1..3) */
+    int integrate_mode;          /* vs_integrate_mode */
 } vs_session_desc;
 
 /* Fill *d with the reference's defaults for a W x H camera and an N^3 cube of
@@ -110,6 +120,11 @@ vs_status vs_session_sync(vs_session *s);
 /* ---- per-stage operators ---------------------------------------------- */
 /* Copy one uint16 mm frame into the session's raw-frame buffer. */
 vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream);
+/* VS_INTEGRATE_COLOR sessions: copy the colour frame registered to the depth
+ * frame, 3 planes (R, G, B) of width*height uint8 as the reference's
+ * g_img_color_t1_, used by the next integration (stream-ordered: give the
+ * stream of the next step, or synchronise). */
+vs_status vs_set_color(vs_session *s, const uint8_t *rgb_planar, int is_host, vs_stream stream);
 /* A2-A3: bilateral filter of the raw frame into pyramid level 0 (metres).
  * Replaces LGKvImageProcessor::bfd_Bilateral_Filter_Depth (z_yooji_cuda_img_proc.cu:394-417). */
 vs_status vs_bilateral(vs_session *s, vs_stream stream);
@@ -156,11 +171,17 @@ vs_status vs_set_filtered(vs_session *s, int level, const float *host);
 /* Model maps written by vs_raycast: depth W_l*H_l, vertex/normal W_l*H_l*3
  * (AoS, camera frame), shaded normal image W_l*H_l. Any pointer may be NULL. */
 vs_status vs_get_maps(vs_session *s, int level, float *depth, float *vtx, float *nrm, uint8_t *img);
+/* VS_INTEGRATE_COLOR sessions: the texture map of the last raycast at `level`,
+ * 3 planes of W_l*H_l uint8 (d_grvi at each hit, 0 elsewhere). */
+vs_status vs_get_texture(vs_session *s, int level, uint8_t *rgb_planar);
 vs_status vs_set_maps(vs_session *s, int level, const float *vtx, const float *nrm);
 /* Whole volume in the reference's linear layout; tsdf/weight converted to
  * float. brick_mask: (W/8)(H/8)(D/8) bytes of the last frame's visibility. */
 vs_status vs_volume_download(vs_session *s, float *tsdf, float *weight, uint8_t *brick_mask);
 vs_status vs_volume_upload(vs_session *s, const float *tsdf, const float *weight);
+/* VS_INTEGRATE_COLOR sessions: the colour volume, 3 planes of W*H*D uint8 in
+ * the linear layout (the reference's GKvObjectCube rgb planes). */
+vs_status vs_volume_download_color(vs_session *s, uint8_t *rgb_planar);
 /* Normal equations of one ICP iteration at `level` for the float pose
  * T_21 (camera t1 -> camera t0): 36 + 6 doubles and the valid count, plus an
  * optional W_l*H_l validity mask. Diagnostic and parity entry point. */

@@ -30,6 +30,13 @@
 //      iterations (a fine-search ray can oscillate forever in the reference).
 //  Q14 bilinear taps that the reference skips (zero residual) contribute 0.
 //  Q15 a Cholesky pivot that is not >= 1e-8 (including NaN) fails the solve.
+//  Q16 Vector3f::toUChar() (ITMMath, not vendored) = ROUND half away from zero,
+//      then clamp to [0, 255] (the ORUtils definition).
+//  Q17 g_ctvc ignores the return of d_gii_Get_Interpolated_Intensity, so a
+//      channel whose bilinear tap fails (outside the image or a participating
+//      intensity <= 0) averages in a stale or uninitialised value; here that
+//      channel keeps its stored value.
+//  Q18 __float2int_rn of a non-finite or |v| >= 2^31 value is "out of range".
 #include "vs_oracle.h"
 
 #include <algorithm>
@@ -552,15 +559,11 @@ int icp_track(const float *const *d1, int W0, int H0, const float *K, int levels
 // ---------------------------------------------------------------------------
 // A13-A14 integrate (z_yooji_cuda_volume_integrator.cu:84-262, 1006-1010)
 // ---------------------------------------------------------------------------
-template <class R>
-void integrate(const uint16_t *mm, int W, int H, const float *K, const float *Tgc, const float *Tcg,
-               const or_volume *vol, R *F, R *Wt, uint8_t *bmask, uint8_t *upd, long long *counts) {
-    const Vol V = make_vol(vol);
-    const size_t nb = (size_t)V.BW * V.BH * V.BD;
-    std::vector<uint8_t> local(nb, 0);
-    uint8_t *mask = bmask ? bmask : local.data();
-    std::memset(mask, 0, nb);
-    // visible bricks: per valid pixel, 6 samples along the ray (n = -4..1)
+// g_fvsc_Find_Visible_Sub_Cubes (:84-152) after the per-frame mask clear
+// (:1006-1010): per valid pixel, 6 samples along the ray (n = -4..1)
+void mark_bricks(const uint16_t *mm, int W, int H, const float *K, const float *Tcg, const Vol &V,
+                 uint8_t *mask) {
+    std::memset(mask, 0, (size_t)V.BW * V.BH * V.BD);
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
@@ -585,6 +588,16 @@ void integrate(const uint16_t *mm, int W, int H, const float *K, const float *Tg
                 mask[((size_t)bxyz[2] * V.BH + bxyz[1]) * V.BW + bxyz[0]] = 1;  // benign race: same value
             }
         }
+}
+
+template <class R>
+void integrate(const uint16_t *mm, int W, int H, const float *K, const float *Tgc, const float *Tcg,
+               const or_volume *vol, R *F, R *Wt, uint8_t *bmask, uint8_t *upd, long long *counts) {
+    const Vol V = make_vol(vol);
+    const size_t nb = (size_t)V.BW * V.BH * V.BD;
+    std::vector<uint8_t> local(nb, 0);
+    uint8_t *mask = bmask ? bmask : local.data();
+    mark_bricks(mm, W, H, K, Tcg, V, mask);
     long long vis = 0, nupd = 0;
     for (size_t b = 0; b < nb; ++b) vis += mask[b];
     const size_t N = (size_t)V.W * V.H * V.D;
@@ -613,6 +626,113 @@ void integrate(const uint16_t *mm, int W, int H, const float *K, const float *Tg
                     R oldW = Wt[idx], newW = oldW + R(1);
                     F[idx] = (oldW * F[idx] + tsd) / newW;
                     Wt[idx] = std::min(newW, R(V.max_w));
+                    if (upd) upd[idx] = 1;
+                    ++nupd;
+                }
+    }
+    if (counts) {
+        counts[0] = vis;
+        counts[1] = nupd;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Production integrate with colour: g_ctvc_Compute_TSDF_of_Voxels_with_Color
+// (z_yooji_cuda_volume_integrator.cu:265-440) as driven by
+// cdtocc_Convert_Depth_to_TSDF_on_Cube_with_Color (:976-1031) with
+// in_flag_on_rgb = true (z_yooji_object_scanner.cpp:1274-1281): the colour
+// frame is registered to the depth frame. Differences from g_ctv: pixel by
+// __float2int_rn, SDF along the ray (depth - |g - C| / sqrt(xn^2 + yn^2 + 1)),
+// voxels at the weight cap are frozen, the average multiplies by 1/newW, and
+// each colour channel is a running average of its bilinear sample
+// (d_gii_Get_Interpolated_Intensity, _yooji_cuda_math.h:305-332), stored as
+// uchar. Colour arithmetic is FP32 in the reference's order (Q0), so colour
+// volumes compare bit for bit.
+// ---------------------------------------------------------------------------
+// __float2int_rn (round half to even), Q18
+inline bool rint_ref(float v, int &out) {
+    if (!(v > -2147483520.0f && v < 2147483520.0f)) return false;
+    out = (int)std::nearbyintf(v);
+    return true;
+}
+// d_gii_Get_Interpolated_Intensity: bilinear over the 2x2 neighbourhood of an
+// 8-bit plane; a participating tap <= 0 or a position outside fails
+inline bool gii(const uint8_t *plane, int W, int H, float xf, float yf, float &out) {
+    if (xf < 0.0f || xf > float(W - 1) || yf < 0.0f || yf > float(H - 1)) return false;
+    float d2 = 0.0f, d3 = 0.0f, d4 = 0.0f;
+    const int x = (int)xf, y = (int)yf;
+    const float rx = xf - (float)x, ry = yf - (float)y;
+    const float d1 = (float)plane[y * W + x];
+    if (d1 <= 0.0f) return false;
+    if (rx > 0.0f) {
+        d2 = (float)plane[y * W + x + 1];
+        if (d2 <= 0.0f) return false;
+    }
+    if (ry > 0.0f) {
+        d3 = (float)plane[(y + 1) * W + x];
+        if (d3 <= 0.0f) return false;
+    }
+    if (rx > 0.0f && ry > 0.0f) {
+        d4 = (float)plane[(y + 1) * W + x + 1];
+        if (d4 <= 0.0f) return false;
+    }
+    out = (1.0f - ry) * ((1.0f - rx) * d1 + rx * d2) + ry * ((1.0f - rx) * d3 + rx * d4);
+    return true;
+}
+
+template <class R>
+void integrate_color(const uint16_t *mm, const uint8_t *rgb, int W, int H, const float *K, const float *Tgc,
+                     const float *Tcg, const or_volume *vol, R *F, R *Wt, uint8_t *C, uint8_t *bmask,
+                     uint8_t *upd, long long *counts) {
+    const Vol V = make_vol(vol);
+    const size_t nb = (size_t)V.BW * V.BH * V.BD;
+    std::vector<uint8_t> local(nb, 0);
+    uint8_t *mask = bmask ? bmask : local.data();
+    mark_bricks(mm, W, H, K, Tcg, V, mask);  // the brick pass is shared with g_ctv
+    long long vis = 0, nupd = 0;
+    for (size_t b = 0; b < nb; ++b) vis += mask[b];
+    const size_t N = (size_t)V.W * V.H * V.D;
+    if (upd) std::memset(upd, 0, N);
+    const f3 cam = {Tcg[3], Tcg[7], Tcg[11]};
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nupd)
+    for (long long b = 0; b < (long long)nb; ++b) {
+        if (!mask[b]) continue;
+        const int bx = (int)(b % V.BW), by = (int)((b / V.BW) % V.BH), bz = (int)(b / ((long long)V.BW * V.BH));
+        for (int zs = 0; zs < 8; ++zs)
+            for (int ys = 0; ys < 8; ++ys)
+                for (int xs = 0; xs < 8; ++xs) {
+                    const int xc = bx * 8 + xs, yc = by * 8 + ys, zc = bz * 8 + zs;
+                    const f3 g = voxel_to_world(V, {(float)xc, (float)yc, (float)zc});
+                    const f3 p = transform(Tgc, g);
+                    float u, v;
+                    project(p, K, u, v);
+                    int x, y;
+                    if (!rint_ref(u, x) || !rint_ref(v, y)) continue;
+                    if (x < 0 || x > W - 1 || y < 0 || y > H - 1) continue;
+                    const float dm = (float)mm[(size_t)y * W + x] * 0.001f;
+                    if (!(dm > 0.0f)) continue;
+                    // lambda: the pixel's ray at unit depth (d_bp_Back_Project of the int pixel)
+                    const f3 r1 = {1.0f * ((float)x - K[2]) / K[0], 1.0f * ((float)y - K[3]) / K[1], 1.0f};
+                    const float lambda_inv = 1.0f / sqrtf(r1.x * r1.x + r1.y * r1.y + 1.0f);
+                    const f3 dg = sub(g, cam);
+                    const float norm = sqrtf(dg.x * dg.x + dg.y * dg.y + dg.z * dg.z);
+                    const float sdf = dm - lambda_inv * norm;
+                    if (sdf < -V.mu) continue;
+                    const size_t idx = ((size_t)zc * V.H + yc) * V.W + xc;
+                    const float oldWf = (float)Wt[idx];
+                    if (oldWf >= V.max_w) continue;  // frozen voxel
+                    const float newWf = oldWf + 1.0f, inewWf = 1.0f / newWf;
+                    const R tsd = std::min(R(1), R(sdf) / R(V.mu));
+                    const R oldW = Wt[idx];
+                    F[idx] = (oldW * F[idx] + tsd) * (R(1) / (oldW + R(1)));
+                    Wt[idx] = oldW + R(1);
+                    for (int k = 0; k < 3; ++k) {
+                        float sample;
+                        if (!gii(rgb + (size_t)k * W * H, W, H, u, v, sample)) continue;  // Q17
+                        const size_t ci = (size_t)k * N + idx;
+                        const float val = (oldWf * (float)C[ci] + sample) * inewWf;
+                        C[ci] = (uint8_t)std::fmax(0.0f, std::fmin(255.0f, val));
+                    }
                     if (upd) upd[idx] = 1;
                     ++nupd;
                 }
@@ -822,9 +942,42 @@ inline bool surface_normal(const Cast &c, f3 p, R n[3]) {
     return true;
 }
 
+// d_grvi_Get_RGB_Value_Interpolated (z_yooji_cuda_tsdf_renderer.cu:232-284)
+// with d_grvu (:192-229) taps: trilinear colour over the 8 corners of
+// toInt(vox); a tap outside the grid or without weight fails the whole
+// lookup, which then leaves (0, 0, 0). Result rounded and clamped (Q16).
+inline void grvi(const Cast &c, const uint8_t *C, f3 vox, uint8_t out[3]) {
+    out[0] = out[1] = out[2] = 0;
+    if (!toint_ok(vox)) return;
+    int o[3];
+    f3 r;
+    to_int(vox, o, r);
+    static const int dx[4] = {0, 1, 0, 1}, dy[4] = {0, 0, 1, 1};
+    const size_t N = (size_t)c.V.W * c.V.H * c.V.D;
+    float fr[4][3], bk[4][3];
+    for (int zz = 0; zz < 2; ++zz)
+        for (int i = 0; i < 4; ++i) {
+            const int x = o[0] + dx[i], y = o[1] + dy[i], z = o[2] + zz;
+            if (x < 0 || x >= c.V.W || y < 0 || y >= c.V.H || z < 0 || z >= c.V.D) return;
+            const size_t idx = ((size_t)z * c.V.H + y) * c.V.W + x;
+            if (!(c.Wt[idx] > 0.0f)) return;
+            for (int k = 0; k < 3; ++k) (zz ? bk : fr)[i][k] = (float)C[k * N + idx];
+        }
+    for (int k = 0; k < 3; ++k) {
+        const float f = (1.0f - r.y) * ((1.0f - r.x) * fr[0][k] + r.x * fr[1][k]) +
+                        r.y * ((1.0f - r.x) * fr[2][k] + r.x * fr[3][k]);
+        const float b = (1.0f - r.y) * ((1.0f - r.x) * bk[0][k] + r.x * bk[1][k]) +
+                        r.y * ((1.0f - r.x) * bk[2][k] + r.x * bk[3][k]);
+        const float v = (1.0f - r.z) * f + r.z * b;
+        const int q = (v < 0.0f) ? (int)(v - 0.5f) : (int)(v + 0.5f);
+        out[k] = (uint8_t)std::min(255, std::max(0, q));
+    }
+}
+
 template <class R>
 void raycast(const float *F, const float *Wt, const or_volume *vol, int W, int H, const float *K, const float *Tgc,
-             const float *Tcg, R *depth, R *vtx, R *nrm, uint8_t *img, long long *stats) {
+             const float *Tcg, R *depth, R *vtx, R *nrm, uint8_t *img, long long *stats, const uint8_t *C = nullptr,
+             uint8_t *tex = nullptr) {
     Cast c{F, Wt, make_vol(vol)};
     const RayParams P = ray_params(c.V, K, Tgc, Tcg);
     long long s0 = 0, s1 = 0, s2 = 0;
@@ -856,6 +1009,13 @@ void raycast(const float *F, const float *Wt, const or_volume *vol, int W, int H
                 R v = (R(0.8) * sa + R(0.2)) * R(255);
                 v = std::max(R(0), std::min(R(255), v));
                 if (img) img[t] = (uint8_t)v;
+                if (tex) {
+                    f3 pv;
+                    world_to_voxel(c.V, p, pv);
+                    uint8_t rgb[3];
+                    grvi(c, C, pv, rgb);
+                    for (int k = 0; k < 3; ++k) tex[(size_t)k * W * H + t] = rgb[k];
+                }
             } else {
                 if (depth) depth[t] = R(0);
                 for (int k = 0; k < 3; ++k) {
@@ -863,6 +1023,8 @@ void raycast(const float *F, const float *Wt, const or_volume *vol, int W, int H
                     if (nrm) nrm[3 * t + k] = R(-100);
                 }
                 if (img) img[t] = 0;
+                if (tex)
+                    for (int k = 0; k < 3; ++k) tex[(size_t)k * W * H + t] = 0;
             }
         }
     }
@@ -1027,6 +1189,16 @@ void or_raycast_d(const float *F, const float *Wt, const or_volume *vol, int W, 
                   const float Tgc[16], const float Tcg[16], double *depth, double *vtx, double *nrm, uint8_t *img,
                   long long stats[4]) {
     raycast<double>(F, Wt, vol, W, H, K, Tgc, Tcg, depth, vtx, nrm, img, stats);
+}
+void or_integrate_color_d(const uint16_t *mm, const uint8_t *rgb, int W, int H, const float K[4], const float Tgc[16],
+                          const float Tcg[16], const or_volume *vol, double *F, double *Wt, uint8_t *C,
+                          uint8_t *bm, uint8_t *upd, long long counts[2]) {
+    integrate_color<double>(mm, rgb, W, H, K, Tgc, Tcg, vol, F, Wt, C, bm, upd, counts);
+}
+void or_raycast_color_d(const float *F, const float *Wt, const uint8_t *C, const or_volume *vol, int W, int H,
+                        const float K[4], const float Tgc[16], const float Tcg[16], double *depth, double *vtx,
+                        double *nrm, uint8_t *img, uint8_t *tex) {
+    raycast<double>(F, Wt, vol, W, H, K, Tgc, Tcg, depth, vtx, nrm, img, nullptr, C, tex);
 }
 void or_raycast_f(const float *F, const float *Wt, const or_volume *vol, int W, int H, const float K[4],
                   const float Tgc[16], const float Tcg[16], float *depth, float *vtx, float *nrm, uint8_t *img,

@@ -90,6 +90,19 @@ void or_raycast_f(const float *F, const float *Wt, const or_volume *vol, int W, 
                   const float Tgc[16], const float Tcg[16], float *depth, float *vtx, float *nrm, uint8_t *img,
                   long long stats[4]);
 
+/* Production integrate with colour (g_ctvc, z_yooji_cuda_volume_integrator.cu:265-440,
+ * registered colour frame as cdtocc(..., flag_on_rgb = true)): rgb = 3 planes of
+ * W*H uint8 (R, G, B) registered to the depth frame; C = 3 planes of the
+ * colour volume (uint8, linear x-fastest). Counts as or_integrate_d. */
+void or_integrate_color_d(const uint16_t *mm, const uint8_t *rgb, int W, int H, const float K[4], const float Tgc[16],
+                          const float Tcg[16], const or_volume *vol, double *F, double *Wt, uint8_t *C,
+                          uint8_t *brick_mask, uint8_t *upd_mask, long long counts[2]);
+/* Raycast with the texture map: tex = 3 planes of W*H uint8 (d_grvi at each hit,
+ * z_yooji_cuda_tsdf_renderer.cu:1166-1184), 0 for misses. */
+void or_raycast_color_d(const float *F, const float *Wt, const uint8_t *C, const or_volume *vol, int W, int H,
+                        const float K[4], const float Tgc[16], const float Tcg[16], double *depth, double *vtx,
+                        double *nrm, uint8_t *img, uint8_t *tex);
+
 /* A19: the whole per-frame pipeline (so_Scan_Object). */
 typedef struct or_session or_session;
 typedef struct {

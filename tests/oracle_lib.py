@@ -225,6 +225,42 @@ def integrate(mm, K, Tgc_f, Tcg_f, vol, F, Wt, fp32=False):
     return bm, upd, counts
 
 
+def integrate_color(mm, rgb, K, Tgc_f, Tcg_f, vol, F, Wt, Cvol):
+    """Production integrate (g_ctvc) with a registered colour frame; rgb is
+    (3, H, W) uint8, Cvol (3, N) uint8 updated in place."""
+    mm = np.ascontiguousarray(mm, np.uint16)
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    H, W = mm.shape
+    assert rgb.shape == (3, H, W) and Cvol.dtype == np.uint8 and Cvol.flags.c_contiguous
+    nb = (vol.dim[0] // 8) * (vol.dim[1] // 8) * (vol.dim[2] // 8)
+    bm = np.zeros(nb, np.uint8)
+    upd = np.zeros(F.shape, np.uint8)
+    counts = np.zeros(2, np.int64)
+    oracle().or_integrate_color_d(_p(mm), _p(rgb), W, H, _p(np.ascontiguousarray(K, np.float32)),
+                                  _p(np.ascontiguousarray(Tgc_f, np.float32)),
+                                  _p(np.ascontiguousarray(Tcg_f, np.float32)), C.byref(vol), _p(F), _p(Wt),
+                                  _p(Cvol), _p(bm), _p(upd), _p(counts))
+    return bm, upd, counts
+
+
+def raycast_color(F, Wt, Cvol, vol, W, H, K, Tgc_f, Tcg_f):
+    """Raycast (FP64 values) with the texture map: returns depth, vtx, nrm,
+    img and tex (3, H, W) uint8."""
+    F = np.ascontiguousarray(F, np.float32)
+    Wt = np.ascontiguousarray(Wt, np.float32)
+    Cvol = np.ascontiguousarray(Cvol, np.uint8)
+    depth = np.zeros((H, W))
+    vtx = np.zeros((H, W, 3))
+    nrm = np.zeros((H, W, 3))
+    img = np.zeros((H, W), np.uint8)
+    tex = np.zeros((3, H, W), np.uint8)
+    oracle().or_raycast_color_d(_p(F), _p(Wt), _p(Cvol), C.byref(vol), W, H, _p(np.ascontiguousarray(K, np.float32)),
+                                _p(np.ascontiguousarray(Tgc_f, np.float32)),
+                                _p(np.ascontiguousarray(Tcg_f, np.float32)), _p(depth), _p(vtx), _p(nrm), _p(img),
+                                _p(tex))
+    return depth, vtx, nrm, img, tex
+
+
 def raycast(F, Wt, vol, W, H, K, Tgc_f, Tcg_f, fp32=False):
     F = np.ascontiguousarray(F, np.float32)
     Wt = np.ascontiguousarray(Wt, np.float32)

@@ -392,6 +392,119 @@ def test_integrate_running_average_of_two_frames():
             assert np.allclose(F3[z][both], (t1 + t2) / 2, atol=1e-12)
 
 
+# ------------------------------------------------- production integrate with colour
+def _color_frame(H, W, rgb):
+    return np.stack([np.full((H, W), c, np.uint8) for c in rgb])
+
+
+def test_integrate_color_constant_frame_and_frozen_voxels():
+    """A constant colour frame leaves that colour in every updated voxel (to
+    within the truncation of the 1/newW product); at the weight cap the voxel
+    freezes: neither its TSDF nor its colour changes again (g_ctvc
+    :391-393), unlike g_ctv, which keeps averaging at the cap."""
+    d = _front_camera()
+    vol = orc.volume(d)
+    K = orc.K_levels(d)[0]
+    N = 32 ** 3
+    F, Wt, Cv = np.ones(N), np.zeros(N), np.zeros((3, N), np.uint8)
+    I4 = np.eye(4, dtype=np.float32)
+    rgb = _color_frame(d.height, d.width, (200, 100, 50))
+    for Zmm in (830, 830, 830):
+        orc.integrate_color(np.full((d.height, d.width), Zmm, np.uint16), rgb, K, I4, I4, vol, F, Wt, Cv)
+    seen = Wt > 0
+    assert seen.sum() > 1000 and np.all(Wt[seen] == 3.0)
+    for k, c in enumerate((200, 100, 50)):
+        # 0 where the unrounded projection leaves [0, W-1] x [0, H-1]: the
+        # colour tap fails there although the rounded depth pixel is inside
+        v = Cv[k][seen]
+        assert np.all((v == c) | (v == c - 1) | (v == 0)) and ((v == c) | (v == c - 1)).mean() > 0.95
+        assert np.all(Cv[k][~seen] == 0)
+    F3, W3, C3 = F.copy(), Wt.copy(), Cv.copy()
+    # two more frames of another plane and colour: frozen at max_weight = 3
+    rgb2 = _color_frame(d.height, d.width, (10, 20, 30))
+    for _ in range(2):
+        orc.integrate_color(np.full((d.height, d.width), 850, np.uint16), rgb2, K, I4, I4, vol, F, Wt, Cv)
+    assert np.array_equal(F[seen], F3[seen]) and np.array_equal(Cv[:, seen], C3[:, seen])
+    assert np.all(Wt <= 3.0)
+
+
+def test_integrate_color_ray_distance_sdf_on_the_optical_axis():
+    """On the pixel ray through the principal point lambda = 1, so the
+    production SDF (depth - |g - C| / lambda) of a voxel centred on the
+    optical axis is depth - |g|. Elsewhere |g - C| / lambda is the voxel's
+    depth along the ray of its rounded pixel, within the half-pixel angle of
+    g_ctv's depth - z (< 1.2 mm here, 0.02 of mu, at 19 deg off axis)."""
+    d = _front_camera()
+    d.cx, d.cy = 160.0, 120.0  # principal point on a pixel centre
+    d.origin[0] = d.origin[1] = -0.33  # voxel column (16, 16) centred on x = y = 0
+    vol = orc.volume(d)
+    K = orc.K_levels(d)[0]
+    N = 32 ** 3
+    F, Wt, Cv = np.ones(N), np.zeros(N), np.zeros((3, N), np.uint8)
+    Fg, Wg = np.ones(N), np.zeros(N)
+    I4 = np.eye(4, dtype=np.float32)
+    mm = np.full((d.height, d.width), 830, np.uint16)
+    orc.integrate_color(mm, _color_frame(d.height, d.width, (9, 9, 9)), K, I4, I4, vol, F, Wt, Cv)
+    orc.integrate(mm, K, I4, I4, vol, Fg, Wg)
+    F3, W3, Fg3 = F.reshape(32, 32, 32), Wt.reshape(32, 32, 32), Fg.reshape(32, 32, 32)
+    depth = float(np.float32(830) * np.float32(0.001))
+    vinv = np.float32(1.0) / np.float32(0.02)
+    c = np.float32((np.float32(16) + np.float32(0.5)) / vinv + np.float32(-0.33))
+    checked = 0
+    for z in range(32):
+        if W3[z, 16, 16] == 0:
+            continue
+        zc = np.float32((np.float32(z) + np.float32(0.5)) / vinv + np.float32(0.5))
+        g = np.array([c, c, zc], np.float64)
+        want = min(1.0, (depth - np.linalg.norm(g)) / float(np.float32(0.06)))
+        assert abs(F3[z, 16, 16] - want) < 2e-6
+        checked += 1
+    assert checked > 3
+    both = (W3 > 0) & (Wg.reshape(32, 32, 32) > 0) & (Fg3 < 1.0) & (F3 < 1.0)
+    assert both.sum() > 1000 and np.abs(F3[both] - Fg3[both]).max() < 0.02
+
+
+def test_integrate_color_failed_channel_keeps_its_value():
+    """Q17: a channel whose bilinear tap reads an intensity of 0 is not
+    updated; the other channels are."""
+    d = _front_camera()
+    vol = orc.volume(d)
+    K = orc.K_levels(d)[0]
+    N = 32 ** 3
+    F, Wt, Cv = np.ones(N), np.zeros(N), np.zeros((3, N), np.uint8)
+    I4 = np.eye(4, dtype=np.float32)
+    mm = np.full((d.height, d.width), 830, np.uint16)
+    orc.integrate_color(mm, _color_frame(d.height, d.width, (80, 0, 160)), K, I4, I4, vol, F, Wt, Cv)
+    seen = Wt > 0
+    inside = seen & (Cv[0] == 80)  # the red tap succeeded: the projection is inside the image
+    assert inside.sum() > 0.95 * seen.sum()
+    assert np.all(Cv[1][seen] == 0) and np.all(Cv[2][inside] == 160)
+
+
+def test_raycast_texture_of_a_uniform_colour_volume(prim_setup):
+    """Trilinear colour of eight equal corners is that colour: every hit of an
+    analytic volume painted (12, 34, 56) renders exactly that; misses are 0."""
+    st, d, F, Wt = prim_setup
+    T = st.pose(2)
+    K = orc.K_levels(d)
+    N = F.size
+    Cv = np.empty((3, N), np.uint8)
+    Cv[0], Cv[1], Cv[2] = 12, 34, 56
+    Tg, Tc = np.float32(T), np.float32(orc.rigid_inverse(T))
+    depth, vtx, nrm, img, tex = orc.raycast_color(F, Wt, Cv, orc.volume(d), d.width, d.height, K[0], Tg, Tc)
+    d2, v2, n2, i2, _ = orc.raycast(F, Wt, orc.volume(d), d.width, d.height, K[0], Tg, Tc)
+    assert np.array_equal(vtx, v2) and np.array_equal(img, i2)
+    hit = vtx[..., 0] != -100
+    assert hit.mean() > 0.5
+    full = np.ones(hit.shape, bool)
+    for k, c in enumerate((12, 34, 56)):
+        # a hit whose 2x2x2 colour stencil touches an unobserved voxel renders 0 (d_grvi failure)
+        assert np.all((tex[k][hit] == c) | (tex[k][hit] == 0))
+        assert np.all(tex[k][~hit] == 0)
+        full &= tex[k] == c
+    assert full[hit].mean() > 0.9
+
+
 # ---------------------------------------------------------------- A16-A18 raycast
 def test_raycast_analytic_scene_within_half_voxel(prim_setup):
     st, d, F, Wt = prim_setup

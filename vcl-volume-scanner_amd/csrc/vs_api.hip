@@ -54,6 +54,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
     size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, lists, params, stats;
+    size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t total;
     size_t nvox;
     uint32_t nbricks, nwords, nsuper, nswords;
@@ -121,6 +122,11 @@ Layout make_layout(const vs_session_desc &d) {
         L.lists = take(lists_bytes(nt, ng, L.nbricks));
     }
     L.stats = take(64);
+    if (d.integrate_mode == VS_INTEGRATE_COLOR) {
+        L.vcol = take(L.nvox * 4);
+        L.rgb = take((size_t)W * H * 3);
+        for (int l = 0; l < kMaxLevels; ++l) L.mtex[l] = take((size_t)(W >> l) * (H >> l) * 3);
+    }
     L.total = off;
     return L;
 }
@@ -139,6 +145,8 @@ vs_status check_desc(const vs_session_desc *d) {
     if (d->weight_type == VS_WEIGHT_U8 && d->max_weight > 255) return fail(VS_EINVAL, "u8 weight cap > 255");
     if (d->tsdf_type < 0 || d->tsdf_type > 1 || d->weight_type < 0 || d->weight_type > 1)
         return fail(VS_EINVAL, "storage type");
+    if (d->integrate_mode != VS_INTEGRATE_DEPTH && d->integrate_mode != VS_INTEGRATE_COLOR)
+        return fail(VS_EINVAL, "integrate_mode %d", d->integrate_mode);
     if (d->bil_radius < 1 || d->bil_radius > 3 || !(d->bil_sigma_s > 0) || !(d->bil_sigma_r > 0))
         return fail(VS_EINVAL, "bilateral parameters");
     if (!(d->fx > 0) || !(d->fy > 0)) return fail(VS_EINVAL, "focal length");
@@ -608,6 +616,12 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.st = (SessState *)(m + L.state);
     P.tsdf_type = d.tsdf_type;
     P.weight_type = d.weight_type;
+    P.integ_mode = d.integrate_mode;
+    if (d.integrate_mode == VS_INTEGRATE_COLOR) {
+        P.vcol = (uint32_t *)(m + L.vcol);
+        P.rgb = (const uint8_t *)(m + L.rgb);
+        for (int l = 0; l < kMaxLevels; ++l) P.mtex[l] = l < d.levels ? (uint8_t *)(m + L.mtex[l]) : nullptr;
+    }
     P.sid = (int)ctx->sessions.size();
     s->st = P.st;
     // self batch
@@ -630,6 +644,7 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
         const size_t n = (size_t)(d.width >> l) * (d.height >> l);
         HIP_TRY(hipMemset(m + L.filt[l], 0, n * 4));
         HIP_TRY(hipMemset(m + L.mimg[l], 0, n));
+        if (P.mtex[l]) HIP_TRY(hipMemset(P.mtex[l], 0, n * 3));
         std::vector<float4> inval(n, make_float4(-100.f, -100.f, -100.f, 0.f));
         HIP_TRY(hipMemcpy(m + L.mvtx[l], inval.data(), n * 16, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(m + L.mnrm[l], inval.data(), n * 16, hipMemcpyHostToDevice));
@@ -686,6 +701,15 @@ vs_status vs_session_desc_get(const vs_session *s, vs_session_desc *out) {
 vs_status vs_session_sync(vs_session *s) {
     if (!s) return fail(VS_EINVAL, "null session");
     return sync_last(s);
+}
+
+vs_status vs_set_color(vs_session *s, const uint8_t *rgb_planar, int is_host, vs_stream stream) {
+    if (!s || !rgb_planar) return fail(VS_EINVAL, "null argument");
+    if (!s->hp.rgb) return fail(VS_EINVAL, "session was not created with VS_INTEGRATE_COLOR");
+    const size_t bytes = (size_t)s->d.width * s->d.height * 3;
+    HIP_TRY(hipMemcpyAsync((void *)s->hp.rgb, rgb_planar, bytes,
+                           is_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, pick(s, stream)));
+    return VS_OK;
 }
 
 vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream) {
@@ -917,6 +941,34 @@ vs_status vs_set_maps(vs_session *s, int level, const float *vtx, const float *n
     }
     HIP_TRY(hipMemcpy(s->hp.mvtx[level], v.data(), n * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->hp.mnrm[level], m.data(), n * 16, hipMemcpyHostToDevice));
+    return VS_OK;
+}
+
+vs_status vs_get_texture(vs_session *s, int level, uint8_t *rgb_planar) {
+    if (!s || !rgb_planar) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    if (!s->hp.mtex[level]) return fail(VS_EINVAL, "session was not created with VS_INTEGRATE_COLOR");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level) * 3;
+    HIP_TRY(hipMemcpy(rgb_planar, s->hp.mtex[level], n, hipMemcpyDeviceToHost));
+    return VS_OK;
+}
+
+vs_status vs_volume_download_color(vs_session *s, uint8_t *rgb_planar) {
+    if (!s || !rgb_planar) return fail(VS_EINVAL, "null argument");
+    if (!s->hp.vcol) return fail(VS_EINVAL, "session was not created with VS_INTEGRATE_COLOR");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = s->L.nvox;
+    uint8_t *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, n * 3);
+    if (e != hipSuccess) return fail(VS_ENOMEM, "download buffer: %s", hipGetErrorString(e));
+    launch_color_to_linear(s->self.dP, n, d, s->stream);
+    e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(rgb_planar, d, n * 3, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (e != hipSuccess) return fail(VS_EHIP, "colour download: %s", hipGetErrorString(e));
     return VS_OK;
 }
 

@@ -97,10 +97,15 @@ struct alignas(16) SessParams {
     uint32_t *socc;          // superbricks (4^3 bricks = 32^3 voxels) holding an occupied brick
     int sdim[3];             // superbricks per axis
     int socc_words;          // 32-bit words of socc
-    double *icp_part;        // ICP block partials (kIcpVals per tile of level 0)
+    double *icp_part;        // ICP group partials (kIcpVals per group of level-0 tiles)
     SessState *st;
     int tsdf_type, weight_type;
     int sid;                 // index in the context's table
+    // VS_INTEGRATE_COLOR sessions (integ_mode 1); null otherwise
+    int integ_mode;
+    uint32_t *vcol;          // brick-major RGB (bits 0-7 R, 8-15 G, 16-23 B)
+    const uint8_t *rgb;      // colour frame: R, G, B planes of the level-0 size
+    uint8_t *mtex[kMaxLevels];  // texture maps: R, G, B planes per level
 };
 
 struct WorkItem {
@@ -144,6 +149,12 @@ __device__ __forceinline__ bool roundf_ref(float v, int &out) {
     return true;
 }
 __device__ __forceinline__ float depth_m(uint16_t mm) { return (float)mm * 0.001f; }
+// __float2int_rn (round half to even); non-finite or |v| >= 2^31 is out of range
+__device__ __forceinline__ bool rint_ref(float v, int &out) {
+    if (!(v > -2147483520.0f && v < 2147483520.0f)) return false;
+    out = (int)rintf(v);
+    return true;
+}
 
 __device__ __forceinline__ bool world_to_voxel(const SessParams &P, f3 p, f3 &vox) {
     vox.x = (p.x - P.origin[0]) * P.vox_inv - 0.5f;

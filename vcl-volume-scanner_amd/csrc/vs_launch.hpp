@@ -69,5 +69,6 @@ void launch_volume_to_linear(const SessParams *P, size_t nvox, int tsdf_type, in
 void launch_volume_from_linear(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, const float *F,
                                const float *Wt, hipStream_t s);
 void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int weight_type, hipStream_t s);
+void launch_color_to_linear(const SessParams *P, size_t nvox, uint8_t *rgb, hipStream_t s);
 
 }  // namespace vs

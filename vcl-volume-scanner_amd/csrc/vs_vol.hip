@@ -113,6 +113,76 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
 // list with a grid stride; 256 threads x 2 voxels = one brick per step.
 // Value and weight are loaded only for voxels that pass the projective test.
 // ---------------------------------------------------------------------------
+// d_gii_Get_Interpolated_Intensity (_yooji_cuda_math.h:305-332): bilinear
+// over the 2x2 neighbourhood of an 8-bit plane; a participating tap <= 0 or
+// a position outside the plane fails.
+__device__ __forceinline__ bool gii(const uint8_t *plane, int W, int H, float xf, float yf, float &out) {
+    if (xf < 0.0f || xf > float(W - 1) || yf < 0.0f || yf > float(H - 1)) return false;
+    float d2 = 0.0f, d3 = 0.0f, d4 = 0.0f;
+    const int x = (int)xf, y = (int)yf;
+    const float rx = xf - (float)x, ry = yf - (float)y;
+    const float d1 = (float)plane[y * W + x];
+    if (d1 <= 0.0f) return false;
+    if (rx > 0.0f) {
+        d2 = (float)plane[y * W + x + 1];
+        if (d2 <= 0.0f) return false;
+    }
+    if (ry > 0.0f) {
+        d3 = (float)plane[(y + 1) * W + x];
+        if (d3 <= 0.0f) return false;
+    }
+    if (rx > 0.0f && ry > 0.0f) {
+        d4 = (float)plane[(y + 1) * W + x + 1];
+        if (d4 <= 0.0f) return false;
+    }
+    out = (1.0f - ry) * ((1.0f - rx) * d1 + rx * d2) + ry * ((1.0f - rx) * d3 + rx * d4);
+    return true;
+}
+
+// One voxel of the production operator g_ctvc_Compute_TSDF_of_Voxels_with_Color
+// (z_yooji_cuda_volume_integrator.cu:265-440, registered colour frame): pixel
+// by __float2int_rn, SDF along the pixel's ray, voxels at the weight cap
+// frozen, average by 1/newW, and each colour channel a running average of its
+// bilinear sample (a channel whose tap fails keeps its value). Returns true
+// when the voxel was updated.
+template <class TV, class TW>
+__device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const float *T, const float *Tcg, f3 g,
+                                                      TV *F, TW *Wt, uint32_t *col, int li) {
+    const LevelDesc &L = P.lev[0];
+    const f3 p = transform(T, g);
+    float u, v;
+    project(p, L.K, u, v);
+    int x, y;
+    if (!rint_ref(u, x) || !rint_ref(v, y)) return false;
+    if (x < 0 || x > L.W - 1 || y < 0 || y > L.H - 1) return false;
+    const float dm = depth_m(P.raw[(size_t)y * L.W + x]);
+    if (!(dm > 0.0f)) return false;
+    const float r1x = 1.0f * ((float)x - L.K[2]) / L.K[0], r1y = 1.0f * ((float)y - L.K[3]) / L.K[1];
+    const float lambda_inv = 1.0f / sqrtf(r1x * r1x + r1y * r1y + 1.0f);
+    const float dx = g.x - Tcg[3], dy = g.y - Tcg[7], dz = g.z - Tcg[11];
+    const float norm = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float sdf = dm - lambda_inv * norm;
+    if (sdf < -P.mu) return false;
+    const float w = ld_w(Wt, li);
+    if (w >= P.max_w) return false;  // frozen voxel
+    const float nw = w + 1.0f, inw = 1.0f / nw;
+    const float tsd = fminf(1.0f, sdf / P.mu);
+    st_val(F, li, (w * ld_val(F, li) + tsd) * inw);
+    st_w(Wt, li, nw);
+    uint32_t c = col[li];
+    const size_t plane = (size_t)L.W * L.H;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float smp;
+        if (!gii(P.rgb + k * plane, L.W, L.H, u, v, smp)) continue;
+        const float val = (w * (float)((c >> (8 * k)) & 255u) + smp) * inw;
+        const uint32_t q = (uint32_t)(uint8_t)fmaxf(0.0f, fminf(255.0f, val));
+        c = (c & ~(255u << (8 * k))) | (q << (8 * k));
+    }
+    col[li] = c;
+    return true;
+}
+
 template <class TV, class TW>
 __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__restrict__ SP, BrickList bl) {
     const unsigned n = min(*bl.count, bl.capacity);
@@ -134,6 +204,15 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
         const float gx = ((float)(bx * kBrick + xl) + 0.5f) / P.vox_inv + P.origin[0];
         const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
         int nupd = 0;
+        if (P.integ_mode == 1) {
+            uint32_t *col = P.vcol + (size_t)b * kBrickVox;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int zc = bz * kBrick + zl + 4 * h;
+                const float gz = ((float)zc + 0.5f) / P.vox_inv + P.origin[2];
+                nupd += integrate_voxel_color(P, T, st->T_cg_f, {gx, gy, gz}, F, Wt, col, t + kTileThreads * h);
+            }
+        } else {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int zc = bz * kBrick + zl + 4 * h;
@@ -155,6 +234,7 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             st_val(F, li, __fdividef(fmaf(w, f, tsd), nw));
             st_w(Wt, li, fminf(nw, P.max_w));
             ++nupd;
+        }
         }
         const int c1 = __syncthreads_count(nupd >= 1), c2 = __syncthreads_count(nupd >= 2);
         if (t == 0 && c1 > 0) {
@@ -499,6 +579,50 @@ __device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n
     return true;
 }
 
+// d_grvi_Get_RGB_Value_Interpolated (z_yooji_cuda_tsdf_renderer.cu:232-284)
+// with its d_grvu taps (:192-229): trilinear colour over the 8 corners of
+// toInt(vox), each channel rounded half away from zero and clamped
+// (Vector3f::toUChar); a corner outside the grid or without weight makes the
+// lookup fail, which leaves (0, 0, 0) (the caller ignores the return,
+// :1171-1175). The stencil is range-checked as a whole and its 16 loads are
+// issued together.
+template <class TV, class TW>
+__device__ __forceinline__ uint32_t rgb_interpolated(const Vol<TV, TW> &V, const uint32_t *col, f3 v) {
+    if (!(fabsf(v.x) < 2147483520.0f && fabsf(v.y) < 2147483520.0f && fabsf(v.z) < 2147483520.0f)) return 0u;
+    const int ox = (int)v.x, oy = (int)v.y, oz = (int)v.z;
+    if (!V.in_grid(ox, oy, oz) || !V.in_grid(ox + 1, oy + 1, oz + 1)) return 0u;
+    const float rx = v.x - (float)ox, ry = v.y - (float)oy, rz = v.z - (float)oz;
+    uint32_t c[8];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const size_t idx = vox_index(V.P, ox + (k & 1), oy + ((k >> 1) & 1), oz + (k >> 2));
+        ok &= ld_w(V.Wt, idx) > 0.0f;
+        c[k] = col[idx];
+    }
+    if (!ok) return 0u;
+    uint32_t out = 0u;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float f[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = (float)((c[k] >> (8 * ch)) & 255u);
+        const float fi = (1.0f - ry) * ((1.0f - rx) * f[0] + rx * f[1]) + ry * ((1.0f - rx) * f[2] + rx * f[3]);
+        const float bi = (1.0f - ry) * ((1.0f - rx) * f[4] + rx * f[5]) + ry * ((1.0f - rx) * f[6] + rx * f[7]);
+        const float r = (1.0f - rz) * fi + rz * bi;
+        const int q = (r < 0.0f) ? (int)(r - 0.5f) : (int)(r + 0.5f);
+        out |= (uint32_t)min(255, max(0, q)) << (8 * ch);
+    }
+    return out;
+}
+
+__device__ __forceinline__ void write_texture(const SessParams &P, int level, size_t t, uint32_t rgb) {
+    const size_t plane = (size_t)P.lev[level].W * P.lev[level].H;
+    P.mtex[level][t] = (uint8_t)(rgb & 255u);
+    P.mtex[level][plane + t] = (uint8_t)((rgb >> 8) & 255u);
+    P.mtex[level][2 * plane + t] = (uint8_t)((rgb >> 16) & 255u);
+}
+
 // The raycast runs as two launches per level. k_raycast marches every ray
 // (g_rmi's d_crtc part) with a small register footprint, so 8 waves per SIMD
 // hide the march's dependent taps; a hit leaves its world point in the vertex
@@ -557,6 +681,7 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
         P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
         P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
         P.mimg[level][t] = 0;
+        if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
     if (STATS) {
         atomicAdd(stats + 0, (unsigned long long)n_near);
@@ -592,11 +717,17 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
         // light (0,0,-1): n.l = 0*nx + 0*ny - nz (z_yooji_object_scanner.cpp:1313)
         const float sa = nc.x * 0.0f + nc.y * 0.0f + nc.z * -1.0f;
         P.mimg[level][t] = (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f)));
+        if (P.mtex[level]) {
+            f3 pv;
+            world_to_voxel(P, p, pv);
+            write_texture(P, level, t, rgb_interpolated(V, P.vcol, pv));
+        }
         if (STATS) atomicAdd(stats + 2, 1ull);
     } else {
         P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
         P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
         P.mimg[level][t] = 0;
+        if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
 }
 
@@ -611,6 +742,7 @@ __global__ void k_volume_init(const SessParams *__restrict__ SP, size_t nvox, ui
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
         st_val(F, i, 1.0f);
         st_w(Wt, i, 0.0f);
+        if (P.vcol) P.vcol[i] = 0u;
     }
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (size_t)gridDim.x * blockDim.x) {
         P.bvis[w] = 0u;
@@ -636,6 +768,22 @@ __global__ void k_volume_to_linear(const SessParams *__restrict__ SP, size_t nvo
         if (Wt) Wt[i] = ld_w((const TW *)P.weight, j);
     }
 }
+// brick-major RGB -> three linear x-fastest planes
+__global__ void k_color_to_linear(const SessParams *__restrict__ SP, size_t nvox, uint8_t *rgb) {
+    const SessParams &P = SP[0];
+    const size_t WH = (size_t)P.vdim[0] * P.vdim[1];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i / WH), y = (int)((i / P.vdim[0]) % P.vdim[1]), x = (int)(i % P.vdim[0]);
+        const uint32_t c = P.vcol[vox_index(P, x, y, z)];
+        rgb[i] = (uint8_t)(c & 255u);
+        rgb[nvox + i] = (uint8_t)((c >> 8) & 255u);
+        rgb[2 * nvox + i] = (uint8_t)((c >> 16) & 255u);
+    }
+}
+void launch_color_to_linear(const SessParams *P, size_t nvox, uint8_t *rgb, hipStream_t s) {
+    hipLaunchKernelGGL(k_color_to_linear, dim3(2048), dim3(256), 0, s, P, nvox, rgb);
+}
+
 template <class TV, class TW>
 __global__ void k_volume_from_linear(const SessParams *__restrict__ SP, size_t nvox, const float *F,
                                      const float *Wt) {

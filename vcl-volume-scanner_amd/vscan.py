@@ -18,6 +18,7 @@ SCENE_PATH = os.path.join(HERE, "libvscan_scene.so")
 VS_OK, VS_EINVAL, VS_ENOMEM, VS_EHIP, VS_TRACK_LOST, VS_MOTION_REJECTED, VS_BUDGET, VS_TOO_FEW_POINTS = range(8)
 VS_TSDF_F32, VS_TSDF_F16 = 0, 1
 VS_WEIGHT_F32, VS_WEIGHT_U8 = 0, 1
+VS_INTEGRATE_DEPTH, VS_INTEGRATE_COLOR = 0, 1
 
 
 class SessionDesc(C.Structure):
@@ -31,6 +32,7 @@ class SessionDesc(C.Structure):
         ("voxel", C.c_float), ("mu", C.c_float), ("max_weight", C.c_float),
         ("tsdf_type", C.c_int), ("weight_type", C.c_int),
         ("bil_sigma_s", C.c_float), ("bil_sigma_r", C.c_float), ("bil_radius", C.c_int),
+        ("integrate_mode", C.c_int),
     ]
 
 
@@ -88,7 +90,8 @@ def lib():
                      "vs_volume_upload", "vs_icp_system", "vs_session_set_timing", "vs_get_stage_ms",
                      "vs_get_integrate_counts", "vs_raycast_counts", "vs_ctx_set_timing", "vs_ctx_stage_ms",
                      "vs_server_create", "vs_server_destroy", "vs_server_open", "vs_server_close",
-                     "vs_server_step", "vs_server_usage"]:
+                     "vs_server_step", "vs_server_usage", "vs_set_color", "vs_get_texture",
+                     "vs_volume_download_color"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
@@ -118,6 +121,9 @@ def lib():
         L.vs_raycast_counts.argtypes = [P, C.c_int, P]
         L.vs_ctx_set_timing.argtypes = [P, C.c_int]
         L.vs_ctx_stage_ms.argtypes = [P, P]
+        L.vs_set_color.argtypes = [P, P, C.c_int, P]
+        L.vs_get_texture.argtypes = [P, C.c_int, P]
+        L.vs_volume_download_color.argtypes = [P, P]
         L.vs_server_create.argtypes = [P, C.POINTER(ServerDesc), C.POINTER(P)]
         L.vs_server_destroy.argtypes = [P]
         L.vs_server_open.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(C.c_int)]
@@ -228,7 +234,7 @@ WORKLOADS = {
 }
 
 
-def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None):
+def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None, integrate_mode=VS_INTEGRATE_DEPTH):
     d = SessionDesc()
     W = W or w["W"]
     H = H or w["H"]
@@ -250,6 +256,7 @@ def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None):
     d.bil_sigma_r = 0.030
     d.bil_radius = 2
     d.icp_dist2 = (0.055 * w["cube"]) ** 2
+    d.integrate_mode = integrate_mode
     return d
 
 
@@ -313,6 +320,24 @@ class Session:
     def nvox(self):
         d = self.desc
         return d.vol_dim[0] * d.vol_dim[1] * d.vol_dim[2]
+
+    def set_color(self, rgb):
+        """rgb: (3, H, W) uint8 planes registered to the depth frame."""
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        assert rgb.shape == (3, self.H, self.W)
+        check(lib().vs_set_color(self.h, _ptr(rgb), 1, None), "vs_set_color")
+
+    def texture(self, level):
+        h, w = self.level_shape(level)
+        out = np.zeros((3, h, w), np.uint8)
+        check(lib().vs_get_texture(self.h, level, _ptr(out)), "vs_get_texture")
+        return out
+
+    def volume_color(self):
+        d = self.desc
+        out = np.zeros((3, d.vol_dim[2], d.vol_dim[1], d.vol_dim[0]), np.uint8)
+        check(lib().vs_volume_download_color(self.h, _ptr(out)), "vs_volume_download_color")
+        return out
 
     def set_depth(self, depth):
         depth = np.ascontiguousarray(depth, np.uint16)
