@@ -206,6 +206,38 @@ vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long
  * the raycast's algorithmic bytes; no counterpart in the reference. */
 vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]);
 
+/* ---- snapshots: save / restore / evict ----------------------------------
+ * Replaces itsdfcd_Import_TSDF_Cube_from_Device + stsdfc_Save_TSDF_Cube
+ * (z_yooji_object_scanner.cpp:2687-2745), whose .vcl files use the closed
+ * KAISION format, with an open sparse format. Layout (little endian):
+ *   vs_snapshot_header
+ *   uint32 brick index [bricks_stored]      (brick (bx,by,bz) = (bz*BH + by)*BW + bx)
+ *   values  [bricks_stored][512]            (value_bytes each: FP32 or FP16)
+ *   weights [bricks_stored][512]            (weight_bytes each: FP32 or uint8)
+ *   colours [bricks_stored][512]            (color_bytes = 4: R, G, B, 0; absent when 0)
+ * Inside a brick voxels are ordered z, y, x (x fastest). Bricks not stored
+ * hold TSDF 1, weight 0, colour 0 (the initial state). A restored session
+ * continues exactly as the saved one would have: its model maps are
+ * re-rendered at the saved pose. */
+typedef struct {
+    char magic[8];           /* "VSCANSNP" */
+    uint32_t version;        /* 1 */
+    uint32_t header_bytes;   /* sizeof(vs_snapshot_header) */
+    vs_session_desc desc;
+    double T_gc[16];
+    int32_t frames;          /* frames fused so far */
+    uint32_t bricks_total;
+    uint32_t bricks_stored;
+    uint32_t value_bytes, weight_bytes, color_bytes;
+} vs_snapshot_header;
+/* Bytes of the session's snapshot now (synchronous). */
+vs_status vs_session_snapshot_bytes(vs_session *s, size_t *bytes);
+/* Write the snapshot into buf (capacity bytes); *bytes = bytes written, or the
+ * size needed when buf is NULL or too small (VS_EINVAL then). Synchronous. */
+vs_status vs_session_snapshot(vs_session *s, void *buf, size_t capacity, size_t *bytes);
+/* A new session of ctx from a snapshot (synchronous). */
+vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_session **out);
+
 /* ---- serving: slot table with continuous batching ------------------------
  * No counterpart in the reference (one scan per process there). A server owns
  * up to max_slots sessions of one storage form. vs_server_open admits a
@@ -236,6 +268,12 @@ vs_status vs_server_step(vs_server *srv, const int *slots, const uint16_t *const
                          vs_stream stream);
 /* Open slots and the session bytes they hold. */
 vs_status vs_server_usage(vs_server *srv, int *open_slots, unsigned long long *bytes);
+/* Eviction and resume: vs_server_evict writes the slot's snapshot into buf
+ * (as vs_session_snapshot) and closes the slot; vs_server_open_snapshot
+ * admits a session from a snapshot under the same slot and budget rules as
+ * vs_server_open. */
+vs_status vs_server_evict(vs_server *srv, int slot, void *buf, size_t capacity, size_t *bytes);
+vs_status vs_server_open_snapshot(vs_server *srv, const void *buf, size_t bytes, int *slot);
 
 #ifdef __cplusplus
 }

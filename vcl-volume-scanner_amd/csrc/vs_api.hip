@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -376,6 +377,39 @@ vs_status server_rebuild(vs_server *v) {
     if (r != VS_OK) return r;
     v->dirty = false;
     return VS_OK;
+}
+
+// Slot and budget check for a session of descriptor d; *slot = a free slot.
+vs_status server_admit(vs_server *v, const vs_session_desc *d, int *slot) {
+    vs_status cs = check_desc(d);
+    if (cs != VS_OK) return cs;
+    if (d->tsdf_type != v->d.tsdf_type || d->weight_type != v->d.weight_type)
+        return fail(VS_EINVAL, "session storage types differ from the server's");
+    const unsigned long long need = make_layout(*d).total;
+    if (v->d.hbm_budget && v->bytes + need > v->d.hbm_budget)
+        return fail(VS_BUDGET, "session needs %llu bytes; %llu of %llu in use", need, v->bytes, v->d.hbm_budget);
+    int i = 0;
+    while (i < (int)v->slot.size() && v->slot[i]) ++i;
+    if (i == (int)v->slot.size()) return fail(VS_BUDGET, "all %d slots open", (int)v->slot.size());
+    *slot = i;
+    return VS_OK;
+}
+
+// Puts a new session into free slot i (destroys it on failure).
+vs_status server_attach(vs_server *v, vs_session *s, int i) {
+    vs_status r = server_quiesce(v);
+    if (r != VS_OK) {
+        vs_session_destroy(s);
+        return r;
+    }
+    v->slot[i] = s;
+    s->server = v;
+    s->slot = i;
+    v->bytes += s->L.total;
+    v->dirty = true;
+    r = server_write_slot(v, i);
+    if (r != VS_OK) vs_session_destroy(s);
+    return r;
 }
 
 hipStream_t pick(vs_session *s, vs_stream st) {
@@ -972,6 +1006,182 @@ vs_status vs_volume_download_color(vs_session *s, uint8_t *rgb_planar) {
     return VS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// snapshots
+// ---------------------------------------------------------------------------
+static void snapshot_sizes(const vs_session_desc &d, uint32_t &sv, uint32_t &sw, uint32_t &sc) {
+    sv = d.tsdf_type == VS_TSDF_F16 ? 2 : 4;
+    sw = d.weight_type == VS_WEIGHT_U8 ? 1 : 4;
+    sc = d.integrate_mode == VS_INTEGRATE_COLOR ? 4 : 0;
+}
+
+// Occupied bricks (bocc bits) of a session, ascending.
+static vs_status occupied_bricks(vs_session *s, std::vector<uint32_t> &out) {
+    std::vector<uint32_t> bits(s->L.nwords);
+    HIP_TRY(hipMemcpy(bits.data(), s->hp.bocc, bits.size() * 4, hipMemcpyDeviceToHost));
+    out.clear();
+    for (uint32_t b = 0; b < s->L.nbricks; ++b)
+        if ((bits[b >> 5] >> (b & 31)) & 1u) out.push_back(b);
+    return VS_OK;
+}
+
+vs_status vs_session_snapshot_bytes(vs_session *s, size_t *bytes) {
+    if (!s || !bytes) return fail(VS_EINVAL, "null argument");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    std::vector<uint32_t> bl;
+    r = occupied_bricks(s, bl);
+    if (r != VS_OK) return r;
+    uint32_t sv, sw, sc;
+    snapshot_sizes(s->d, sv, sw, sc);
+    *bytes = sizeof(vs_snapshot_header) + bl.size() * (4 + (size_t)kBrickVox * (sv + sw + sc));
+    return VS_OK;
+}
+
+vs_status vs_session_snapshot(vs_session *s, void *buf, size_t capacity, size_t *bytes) {
+    if (!s || !bytes) return fail(VS_EINVAL, "null argument");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    std::vector<uint32_t> bl;
+    r = occupied_bricks(s, bl);
+    if (r != VS_OK) return r;
+    uint32_t sv, sw, sc;
+    snapshot_sizes(s->d, sv, sw, sc);
+    const size_t n = bl.size(), packed = n * (size_t)kBrickVox * (sv + sw + sc);
+    const size_t need = sizeof(vs_snapshot_header) + n * 4 + packed;
+    *bytes = need;
+    if (!buf || capacity < need) return buf ? fail(VS_EINVAL, "snapshot needs %zu bytes, buffer has %zu", need, capacity)
+                                            : VS_OK;
+    SessState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof(st), hipMemcpyDeviceToHost));
+    vs_snapshot_header h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "VSCANSNP", 8);
+    h.version = 1;
+    h.header_bytes = sizeof(h);
+    h.desc = s->d;
+    std::memcpy(h.T_gc, st.T_gc, sizeof(h.T_gc));
+    h.frames = st.frames;
+    h.bricks_total = s->L.nbricks;
+    h.bricks_stored = (uint32_t)n;
+    h.value_bytes = sv;
+    h.weight_bytes = sw;
+    h.color_bytes = sc;
+    char *out = (char *)buf;
+    std::memcpy(out, &h, sizeof(h));
+    std::memcpy(out + sizeof(h), bl.data(), n * 4);
+    if (n) {
+        char *dev = nullptr;
+        hipError_t e = hipMalloc((void **)&dev, align_up(n * 4) + packed);
+        if (e != hipSuccess) return fail(VS_ENOMEM, "snapshot buffer: %s", hipGetErrorString(e));
+        e = hipMemcpy(dev, bl.data(), n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_brick_pack(s->self.dP, (const uint32_t *)dev, (unsigned)n, (int)sv, (int)sw, (int)sc,
+                              (uint8_t *)dev + align_up(n * 4), 0, s->stream);
+            e = hipStreamSynchronize(s->stream);
+        }
+        if (e == hipSuccess) e = hipMemcpy(out + sizeof(h) + n * 4, dev + align_up(n * 4), packed, hipMemcpyDeviceToHost);
+        hipFree(dev);
+        if (e != hipSuccess) return fail(VS_EHIP, "snapshot: %s", hipGetErrorString(e));
+    }
+    return VS_OK;
+}
+
+// Header checks of a snapshot of `bytes` bytes; returns its descriptor.
+static vs_status snapshot_check(const void *buf, size_t bytes, vs_snapshot_header &h) {
+    if (!buf || bytes < sizeof(vs_snapshot_header)) return fail(VS_EINVAL, "snapshot too short");
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, "VSCANSNP", 8) != 0 || h.version != 1 || h.header_bytes != sizeof(h))
+        return fail(VS_EINVAL, "not a version-1 snapshot");
+    vs_status cs = check_desc(&h.desc);
+    if (cs != VS_OK) return cs;
+    uint32_t sv, sw, sc;
+    snapshot_sizes(h.desc, sv, sw, sc);
+    const Layout L = make_layout(h.desc);
+    if (h.value_bytes != sv || h.weight_bytes != sw || h.color_bytes != sc || h.bricks_total != L.nbricks ||
+        h.bricks_stored > L.nbricks)
+        return fail(VS_EINVAL, "snapshot sizes do not match its descriptor");
+    const size_t need = sizeof(h) + (size_t)h.bricks_stored * (4 + (size_t)kBrickVox * (sv + sw + sc));
+    if (bytes < need) return fail(VS_EINVAL, "snapshot truncated (%zu of %zu bytes)", bytes, need);
+    const uint32_t *idx = (const uint32_t *)((const char *)buf + sizeof(h));
+    for (uint32_t i = 0; i < h.bricks_stored; ++i)
+        if (idx[i] >= L.nbricks) return fail(VS_EINVAL, "snapshot brick index %u out of range", idx[i]);
+    return VS_OK;
+}
+
+vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_session **out) {
+    if (!ctx || !out) return fail(VS_EINVAL, "null argument");
+    vs_snapshot_header h;
+    vs_status r = snapshot_check(buf, bytes, h);
+    if (r != VS_OK) return r;
+    vs_session *s = nullptr;
+    r = vs_session_create(ctx, &h.desc, h.T_gc, &s);
+    if (r != VS_OK) return r;
+    const size_t n = h.bricks_stored;
+    const size_t packed = n * (size_t)kBrickVox * (h.value_bytes + h.weight_bytes + h.color_bytes);
+    hipError_t e = hipSuccess;
+    if (n) {
+        char *dev = nullptr;
+        e = hipMalloc((void **)&dev, align_up(n * 4) + packed);
+        if (e == hipSuccess) {
+            const char *in = (const char *)buf + sizeof(h);
+            e = hipMemcpy(dev, in, n * 4, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(dev + align_up(n * 4), in + n * 4, packed, hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                launch_brick_pack(s->self.dP, (const uint32_t *)dev, (unsigned)n, (int)h.value_bytes,
+                                  (int)h.weight_bytes, (int)h.color_bytes, (uint8_t *)dev + align_up(n * 4), 1,
+                                  s->stream);
+                launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.weight_type, s->stream);
+                e = hipStreamSynchronize(s->stream);
+            }
+            hipFree(dev);
+        }
+    }
+    if (e == hipSuccess) {
+        const int32_t frames = h.frames;
+        e = hipMemcpy((char *)s->st + offsetof(SessState, frames), &frames, sizeof(frames), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        vs_session_destroy(s);
+        return fail(VS_EHIP, "restore: %s", hipGetErrorString(e));
+    }
+    // the model maps the next frame tracks against: render them at the saved pose
+    for (int l = 0; l < s->d.levels && r == VS_OK; ++l) r = vs_raycast(s, l, nullptr);
+    if (r == VS_OK) r = sync_last(s);
+    if (r != VS_OK) {
+        vs_session_destroy(s);
+        return r;
+    }
+    *out = s;
+    return VS_OK;
+}
+
+vs_status vs_server_evict(vs_server *v, int slot, void *buf, size_t capacity, size_t *bytes) {
+    if (!v) return fail(VS_EINVAL, "null server");
+    if (slot < 0 || slot >= (int)v->slot.size() || !v->slot[slot]) return fail(VS_EINVAL, "slot %d not open", slot);
+    vs_status r = server_quiesce(v);
+    if (r == VS_OK) r = vs_session_snapshot(v->slot[slot], buf, capacity, bytes);
+    if (r != VS_OK || !buf) return r;
+    return vs_server_close(v, slot);
+}
+
+vs_status vs_server_open_snapshot(vs_server *v, const void *buf, size_t bytes, int *slot) {
+    if (!v || !slot) return fail(VS_EINVAL, "null argument");
+    vs_snapshot_header h;
+    vs_status r = snapshot_check(buf, bytes, h);
+    if (r != VS_OK) return r;
+    int i = -1;
+    r = server_admit(v, &h.desc, &i);
+    if (r != VS_OK) return r;
+    vs_session *s = nullptr;
+    r = vs_session_restore(v->ctx, buf, bytes, &s);
+    if (r != VS_OK) return r;
+    r = server_attach(v, s, i);
+    if (r != VS_OK) return r;
+    *slot = i;
+    return VS_OK;
+}
+
 vs_status vs_volume_download(vs_session *s, float *tsdf, float *weight, uint8_t *brick_mask) {
     if (!s) return fail(VS_EINVAL, "null session");
     vs_status r = sync_last(s);
@@ -1173,34 +1383,14 @@ vs_status vs_server_destroy(vs_server *v) {
 
 vs_status vs_server_open(vs_server *v, const vs_session_desc *desc, const double T_gc0[16], int *slot) {
     if (!v || !desc || !slot) return fail(VS_EINVAL, "null argument");
-    vs_status cs = check_desc(desc);
-    if (cs != VS_OK) return cs;
-    if (desc->tsdf_type != v->d.tsdf_type || desc->weight_type != v->d.weight_type)
-        return fail(VS_EINVAL, "session storage types differ from the server's");
-    const unsigned long long need = make_layout(*desc).total;
-    if (v->d.hbm_budget && v->bytes + need > v->d.hbm_budget)
-        return fail(VS_BUDGET, "session needs %llu bytes; %llu of %llu in use", need, v->bytes, v->d.hbm_budget);
-    int i = 0;
-    while (i < (int)v->slot.size() && v->slot[i]) ++i;
-    if (i == (int)v->slot.size()) return fail(VS_BUDGET, "all %d slots open", (int)v->slot.size());
-    vs_session *s = nullptr;
-    vs_status r = vs_session_create(v->ctx, desc, T_gc0, &s);
+    int i = -1;
+    vs_status r = server_admit(v, desc, &i);
     if (r != VS_OK) return r;
-    r = server_quiesce(v);
-    if (r != VS_OK) {
-        vs_session_destroy(s);
-        return r;
-    }
-    v->slot[i] = s;
-    s->server = v;
-    s->slot = i;
-    v->bytes += s->L.total;
-    v->dirty = true;
-    r = server_write_slot(v, i);
-    if (r != VS_OK) {
-        vs_session_destroy(s);
-        return r;
-    }
+    vs_session *s = nullptr;
+    r = vs_session_create(v->ctx, desc, T_gc0, &s);
+    if (r != VS_OK) return r;
+    r = server_attach(v, s, i);
+    if (r != VS_OK) return r;
     *slot = i;
     return VS_OK;
 }

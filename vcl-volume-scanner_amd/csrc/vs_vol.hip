@@ -768,6 +768,41 @@ __global__ void k_volume_to_linear(const SessParams *__restrict__ SP, size_t nvo
         if (Wt) Wt[i] = ld_w((const TW *)P.weight, j);
     }
 }
+// Whole bricks between the volume planes and a packed array (snapshots):
+// values, then weights, then colours of the listed bricks, one workgroup per
+// brick, 16-byte vectors (every offset is a multiple of 512 bytes).
+__device__ __forceinline__ void copy16(uint8_t *dst, const uint8_t *src, size_t bytes) {
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    for (size_t k = threadIdx.x; k < bytes / 16; k += blockDim.x) d[k] = s[k];
+}
+__global__ void k_brick_pack(const SessParams *__restrict__ SP, const uint32_t *idx, unsigned n, int sv, int sw,
+                             int sc, uint8_t *out, int unpack) {
+    const SessParams &P = SP[0];
+    const size_t bv = (size_t)kBrickVox * sv, bw = (size_t)kBrickVox * sw, bc = (size_t)kBrickVox * sc;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const size_t b = idx[i];
+        uint8_t *pv = out + i * bv, *pw = out + n * bv + i * bw, *pc = out + n * (bv + bw) + i * bc;
+        uint8_t *vv = (uint8_t *)P.tsdf + b * bv, *vw = (uint8_t *)P.weight + b * bw;
+        uint8_t *vc = sc ? (uint8_t *)P.vcol + b * bc : nullptr;
+        if (unpack) {
+            copy16(vv, pv, bv);
+            copy16(vw, pw, bw);
+            if (sc) copy16(vc, pc, bc);
+        } else {
+            copy16(pv, vv, bv);
+            copy16(pw, vw, bw);
+            if (sc) copy16(pc, vc, bc);
+        }
+    }
+}
+void launch_brick_pack(const SessParams *P, const uint32_t *idx, unsigned n, int sv, int sw, int sc, uint8_t *buf,
+                       int unpack, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_brick_pack, dim3(n < 4096u ? n : 4096u), dim3(256), 0, s, P, idx, n, sv, sw, sc, buf,
+                           unpack);
+}
+
 // brick-major RGB -> three linear x-fastest planes
 __global__ void k_color_to_linear(const SessParams *__restrict__ SP, size_t nvox, uint8_t *rgb) {
     const SessParams &P = SP[0];

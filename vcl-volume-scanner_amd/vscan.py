@@ -36,6 +36,13 @@ class SessionDesc(C.Structure):
     ]
 
 
+class SnapshotHeader(C.Structure):
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("header_bytes", C.c_uint32),
+                ("desc", SessionDesc), ("T_gc", C.c_double * 16), ("frames", C.c_int32),
+                ("bricks_total", C.c_uint32), ("bricks_stored", C.c_uint32), ("value_bytes", C.c_uint32),
+                ("weight_bytes", C.c_uint32), ("color_bytes", C.c_uint32)]
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("max_slots", C.c_int), ("tsdf_type", C.c_int), ("weight_type", C.c_int),
                 ("hbm_budget", C.c_ulonglong)]
@@ -91,7 +98,8 @@ def lib():
                      "vs_get_integrate_counts", "vs_raycast_counts", "vs_ctx_set_timing", "vs_ctx_stage_ms",
                      "vs_server_create", "vs_server_destroy", "vs_server_open", "vs_server_close",
                      "vs_server_step", "vs_server_usage", "vs_set_color", "vs_get_texture",
-                     "vs_volume_download_color"]:
+                     "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
+                     "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
@@ -122,6 +130,11 @@ def lib():
         L.vs_ctx_set_timing.argtypes = [P, C.c_int]
         L.vs_ctx_stage_ms.argtypes = [P, P]
         L.vs_set_color.argtypes = [P, P, C.c_int, P]
+        L.vs_session_snapshot_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.vs_session_snapshot.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.vs_session_restore.argtypes = [P, P, C.c_size_t, C.POINTER(P)]
+        L.vs_server_evict.argtypes = [P, C.c_int, P, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.vs_server_open_snapshot.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_int)]
         L.vs_get_texture.argtypes = [P, C.c_int, P]
         L.vs_volume_download_color.argtypes = [P, P]
         L.vs_server_create.argtypes = [P, C.POINTER(ServerDesc), C.POINTER(P)]
@@ -321,6 +334,14 @@ class Session:
         d = self.desc
         return d.vol_dim[0] * d.vol_dim[1] * d.vol_dim[2]
 
+    def snapshot(self):
+        """The session's snapshot (the open sparse format of vscan.h) as bytes."""
+        n = C.c_size_t()
+        check(lib().vs_session_snapshot_bytes(self.h, C.byref(n)), "vs_session_snapshot_bytes")
+        buf = np.zeros(n.value, np.uint8)
+        check(lib().vs_session_snapshot(self.h, _ptr(buf), n.value, C.byref(n)), "vs_session_snapshot")
+        return buf[:n.value].tobytes()
+
     def set_color(self, rgb):
         """rgb: (3, H, W) uint8 planes registered to the depth frame."""
         rgb = np.ascontiguousarray(rgb, np.uint8)
@@ -447,6 +468,17 @@ class Session:
         return c
 
 
+def restore(ctx, data):
+    """A new Session from snapshot bytes."""
+    buf = np.frombuffer(data, np.uint8)
+    h = P()
+    check(lib().vs_session_restore(ctx.h, _ptr(buf), len(data), C.byref(h)), "vs_session_restore")
+    hdr = SnapshotHeader.from_buffer_copy(data[:C.sizeof(SnapshotHeader)])
+    s = Session(ctx, hdr.desc, handle=h.value)
+    s.owned = True
+    return s
+
+
 def session_bytes(desc):
     return lib().vs_session_bytes(C.byref(desc))
 
@@ -494,6 +526,28 @@ class Server:
         keep = [np.ascontiguousarray(d, np.uint16) if is_host else d for d in depths]
         ds = (P * n)(*[(_ptr(k) if is_host else P(k)) for k in keep])
         check(lib().vs_server_step(self.h, _ptr(sl), ds, n, 1 if is_host else 0, stream), "vs_server_step")
+
+    def evict(self, slot):
+        """Snapshot bytes of the slot's session; the slot is closed."""
+        n = C.c_size_t()
+        check(lib().vs_server_evict(self.h, slot, None, 0, C.byref(n)), "vs_server_evict")
+        buf = np.zeros(n.value, np.uint8)
+        s = self.sessions.pop(slot)
+        s.h = P()
+        check(lib().vs_server_evict(self.h, slot, _ptr(buf), n.value, C.byref(n)), "vs_server_evict")
+        return buf[:n.value].tobytes()
+
+    def open_snapshot(self, data):
+        """Slot of the session restored from snapshot bytes, or None (VS_BUDGET)."""
+        buf = np.frombuffer(data, np.uint8)
+        slot = C.c_int(-1)
+        code = lib().vs_server_open_snapshot(self.h, _ptr(buf), len(data), C.byref(slot))
+        if code == VS_BUDGET:
+            return None
+        check(code, "vs_server_open_snapshot")
+        hdr = SnapshotHeader.from_buffer_copy(data[:C.sizeof(SnapshotHeader)])
+        self.sessions[slot.value] = Session(self.ctx, hdr.desc, handle=lib().vs_server_session(self.h, slot.value))
+        return slot.value
 
     def usage(self):
         n, b = C.c_int(), C.c_ulonglong()
