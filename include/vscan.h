@@ -153,12 +153,39 @@ vs_status vs_integrate(vs_session *s, vs_stream stream);
 vs_status vs_raycast(vs_session *s, int level, vs_stream stream);
 /* A19: one fused frame (filter -> pyramid -> track (frames > 0) -> integrate
  * raw depth -> raycast every level), as so_Scan_Object
- * (z_yooji_object_scanner.cpp:472-629). Asynchronous; see vs_get_frame_status. */
+ * (z_yooji_object_scanner.cpp:472-629). Asynchronous; see vs_get_frame_status.
+ * depth_mm NULL fuses the frame already in the session's raw buffer
+ * (vs_set_depth or a pre-step). */
 vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream);
 /* Many sessions, one frame each, in one batched launch per stage. Sessions
- * must share tsdf/weight types. depths[i] are device pointers unless is_host. */
+ * must share tsdf/weight types. depths[i] are device pointers unless is_host;
+ * a NULL depths[i] (or depths NULL) fuses the frame already in that session's
+ * raw buffer. */
 vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t *const *depths,
                         int n, int is_host, vs_stream stream);
+
+/* ---- input pre-steps ------------------------------------------------------
+ * The CPU steps either side of the filter in the reference's input path, on
+ * the device. Depth buffers are device pointers: uint16 millimetres, or float
+ * metres when *_is_float. A NULL frame pointer selects the session's own
+ * raw-frame buffer (uint16, the session's size and intrinsics), which a
+ * following vs_fuse_frame / vs_fuse_batch / vs_server_step with a NULL depth
+ * then fuses as it is. */
+/* z_tdrcc_Transform_Depth_to_Rgb_Camera_Coordinates (z_yooji_object_scanner.cpp:2429-2476, with
+ * CKvYooji_MatrixRgbD::convert_pixel_coord, z_yooji_matrix_rgbd.cpp:101-118): forward-project the
+ * source frame (src_w x src_h, intrinsics K_src) through T_dst_src (row-major, source camera to the
+ * session's camera) into a frame of the session's size and intrinsics, keeping the nearest source
+ * depth per pixel and 0 where none lands. dst_is_float must equal src_is_float (the depth value is
+ * moved, not converted). */
+vs_status vs_depth_reproject(vs_session *s, const void *src, int src_w, int src_h, const float K_src[4],
+                             const float T_dst_src[16], int src_is_float, void *dst, vs_stream stream);
+/* god_Get_Object_Depth (z_yooji_object_extractor.cpp:357-410), its volume test: zero, in place,
+ * every depth whose back-projection (intrinsics K) lies outside the session's volume box
+ * [origin, origin + dim*voxel] in the world, using T_cg (camera to world, row-major) or, when NULL,
+ * the session's current pose as the device holds it (stream-ordered). The hand-colour
+ * segmentation of that function is not part of this operator. */
+vs_status vs_depth_crop_to_volume(vs_session *s, void *depth, int w, int h, const float K[4], const float *T_cg,
+                                  int is_float, vs_stream stream);
 
 /* ---- state access (synchronous unless noted) --------------------------- */
 /* Status of the last tracked frame (vs_status value) and frames fused so far. */

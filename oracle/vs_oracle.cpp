@@ -974,6 +974,48 @@ inline void grvi(const Cast &c, const uint8_t *C, f3 vox, uint8_t out[3]) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Input pre-steps. z_tdrcc_Transform_Depth_to_Rgb_Camera_Coordinates
+// (z_yooji_object_scanner.cpp:2429-2476) with convert_pixel_coord
+// (z_yooji_matrix_rgbd.cpp:101-118) and the CKvYooji_Intrinsics /
+// CKvYooji_Extrinsics maths (z_yooji_camera_parameters.cpp:89-108, 678-694);
+// god_Get_Object_Depth's cube test (z_yooji_object_extractor.cpp:357-410).
+// Both move or zero depth values without changing them, so they run on the
+// uint16 millimetres (depth = mm * 0.001f as A1).
+// ---------------------------------------------------------------------------
+void depth_reproject(const uint16_t *src, int sw, int sh, const float *Ks, const float *T, uint16_t *dst, int dw,
+                     int dh, const float *Kd) {
+    std::memset(dst, 0, sizeof(uint16_t) * (size_t)dw * dh);
+    for (int j = 0; j < sh; ++j)
+        for (int i = 0; i < sw; ++i) {
+            const uint16_t raw = src[(size_t)j * sw + i];
+            const float td = (float)raw * 0.001f;
+            if (!(td > 0.0f)) continue;
+            const f3 p = {td * ((float)i - Ks[2]) / Ks[0], td * ((float)j - Ks[3]) / Ks[1], td};
+            const f3 q = transform(T, p);
+            const float u = Kd[0] * q.x / q.z + Kd[2], v = Kd[1] * q.y / q.z + Kd[3];
+            int tx, ty;
+            if (!roundf_ref(u, tx) || !roundf_ref(v, ty)) continue;
+            if (tx < 0 || tx > dw - 1 || ty < 0 || ty > dh - 1) continue;
+            uint16_t &o = dst[(size_t)ty * dw + tx];
+            // "if(out > td || out <= 0) out = td": the nearest depth wins
+            if ((float)o * 0.001f > td || o == 0) o = raw;
+        }
+}
+
+void depth_crop(uint16_t *mm, int W, int H, const float *K, const float *Tcg, const Vol &V) {
+    const float sx = (float)V.W * V.voxel, sy = (float)V.H * V.voxel, sz = (float)V.D * V.voxel;
+    for (int j = 0; j < H; ++j)
+        for (int i = 0; i < W; ++i) {
+            const float td = (float)mm[(size_t)j * W + i] * 0.001f;
+            const f3 p = {td * ((float)i - K[2]) / K[0], td * ((float)j - K[3]) / K[1], td};
+            const f3 g = transform(Tcg, p);
+            if (g.x < V.origin.x || g.x > V.origin.x + sx || g.y < V.origin.y || g.y > V.origin.y + sy ||
+                g.z < V.origin.z || g.z > V.origin.z + sz)
+                mm[(size_t)j * W + i] = 0;
+        }
+}
+
 template <class R>
 void raycast(const float *F, const float *Wt, const or_volume *vol, int W, int H, const float *K, const float *Tgc,
              const float *Tcg, R *depth, R *vtx, R *nrm, uint8_t *img, long long *stats, const uint8_t *C = nullptr,
@@ -1199,6 +1241,13 @@ void or_raycast_color_d(const float *F, const float *Wt, const uint8_t *C, const
                         const float K[4], const float Tgc[16], const float Tcg[16], double *depth, double *vtx,
                         double *nrm, uint8_t *img, uint8_t *tex) {
     raycast<double>(F, Wt, vol, W, H, K, Tgc, Tcg, depth, vtx, nrm, img, nullptr, C, tex);
+}
+void or_depth_reproject(const uint16_t *src, int sw, int sh, const float K_src[4], const float T_dst_src[16],
+                        uint16_t *dst, int dw, int dh, const float K_dst[4]) {
+    depth_reproject(src, sw, sh, K_src, T_dst_src, dst, dw, dh, K_dst);
+}
+void or_depth_crop(uint16_t *mm, int W, int H, const float K[4], const float T_cg[16], const or_volume *vol) {
+    depth_crop(mm, W, H, K, T_cg, make_vol(vol));
 }
 void or_raycast_f(const float *F, const float *Wt, const or_volume *vol, int W, int H, const float K[4],
                   const float Tgc[16], const float Tcg[16], float *depth, float *vtx, float *nrm, uint8_t *img,

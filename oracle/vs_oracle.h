@@ -103,6 +103,13 @@ void or_raycast_color_d(const float *F, const float *Wt, const uint8_t *C, const
                         const float K[4], const float Tgc[16], const float Tcg[16], double *depth, double *vtx,
                         double *nrm, uint8_t *img, uint8_t *tex);
 
+/* Input pre-steps on uint16 mm frames: z_tdrcc reprojection into the
+ * destination camera (nearest depth wins, 0 where none lands) and the
+ * god_Get_Object_Depth volume crop (in place). */
+void or_depth_reproject(const uint16_t *src, int sw, int sh, const float K_src[4], const float T_dst_src[16],
+                        uint16_t *dst, int dw, int dh, const float K_dst[4]);
+void or_depth_crop(uint16_t *mm, int W, int H, const float K[4], const float T_cg[16], const or_volume *vol);
+
 /* A19: the whole per-frame pipeline (so_Scan_Object). */
 typedef struct or_session or_session;
 typedef struct {

@@ -243,6 +243,24 @@ def integrate_color(mm, rgb, K, Tgc_f, Tcg_f, vol, F, Wt, Cvol):
     return bm, upd, counts
 
 
+def depth_reproject(src, K_src, T_dst_src, dst_shape, K_dst):
+    src = np.ascontiguousarray(src, np.uint16)
+    sh, sw = src.shape
+    out = np.zeros(dst_shape, np.uint16)
+    oracle().or_depth_reproject(_p(src), sw, sh, _p(np.ascontiguousarray(K_src, np.float32)),
+                                _p(np.ascontiguousarray(T_dst_src, np.float32)), _p(out), dst_shape[1], dst_shape[0],
+                                _p(np.ascontiguousarray(K_dst, np.float32)))
+    return out
+
+
+def depth_crop(mm, K, T_cg_f, vol):
+    out = np.ascontiguousarray(mm, np.uint16).copy()
+    H, W = out.shape
+    oracle().or_depth_crop(_p(out), W, H, _p(np.ascontiguousarray(K, np.float32)),
+                           _p(np.ascontiguousarray(T_cg_f, np.float32)), C.byref(vol))
+    return out
+
+
 def raycast_color(F, Wt, Cvol, vol, W, H, K, Tgc_f, Tcg_f):
     """Raycast (FP64 values) with the texture map: returns depth, vtx, nrm,
     img and tex (3, H, W) uint8."""

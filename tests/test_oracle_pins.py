@@ -505,6 +505,64 @@ def test_raycast_texture_of_a_uniform_colour_volume(prim_setup):
     assert full[hit].mean() > 0.9
 
 
+# ------------------------------------------------------------------- input pre-steps
+def test_depth_reproject_identity_and_integer_shift():
+    """Same camera: the frame comes back unchanged. A translation of the
+    destination camera by t along x moves a fronto-parallel plane at depth Z
+    by exactly fx * t / Z pixels when that is an integer."""
+    K = np.array([262.5, 262.5, 159.5, 119.5], np.float32)
+    rng = np.random.default_rng(5)
+    mm = rng.integers(400, 2000, size=(240, 320)).astype(np.uint16)
+    mm[rng.random((240, 320)) < 0.05] = 0
+    I4 = np.eye(4, dtype=np.float32)
+    assert np.array_equal(orc.depth_reproject(mm, K, I4, (240, 320), K), mm)
+    Z = 1050  # mm; fx * t / Z = 262.5 * 0.02 / 1.05 = 5 pixels
+    plane = np.full((240, 320), Z, np.uint16)
+    T = np.eye(4, dtype=np.float32)
+    T[0, 3] = np.float32(-0.02)  # the destination camera sits 2 cm to the right
+    out = orc.depth_reproject(plane, K, T, (240, 320), K)
+    assert np.all(out[:, :315] == Z) and np.all(out[:, 315:] == 0)
+
+
+def test_depth_reproject_keeps_the_nearest_depth():
+    """Two source pixels landing on one destination pixel: the nearer depth is
+    kept whichever is visited first."""
+    K = np.array([100.0, 100.0, 8.0, 8.0], np.float32)
+    src = np.zeros((17, 17), np.uint16)
+    src[8, 8] = 1000
+    src[8, 9] = 800  # at 0.8 m, 1 px right: x = 0.008
+    T = np.eye(4, dtype=np.float32)
+    T[0, 3] = np.float32(-0.008)  # moves the 0.8 m point onto the principal point
+    out = orc.depth_reproject(src, K, T, (17, 17), K)
+    assert out[8, 8] == 800 and out[8, 0] == 0
+    src2 = src.copy()
+    src2[8, 8], src2[8, 9] = 0, 800
+    assert orc.depth_reproject(src2, K, T, (17, 17), K)[8, 8] == 800
+
+
+def test_depth_crop_closed_form():
+    """A fronto-parallel plane: depths whose point lies inside the volume box
+    survive, the rest are zeroed (columns outside the box's x/y extent, and
+    every pixel when the plane lies beyond the box)."""
+    d = _front_camera()
+    vol = orc.volume(d)
+    K = orc.K_levels(d)[0]
+    I4 = np.eye(4, dtype=np.float32)
+    mm = np.full((d.height, d.width), 830, np.uint16)
+    out = orc.depth_crop(mm, K, I4, vol)
+    # box: x, y in [-0.32, 0.32], z in [0.5, 1.14]; X = 0.83 (u - cx) / fx
+    u = np.arange(d.width, dtype=np.float32)
+    X = np.float32(0.83) * (u - np.float32(K[2])) / np.float32(K[0])
+    inside_u = (X >= np.float32(-0.32)) & (X <= np.float32(-0.32) + np.float32(32 * 0.02))
+    v = np.arange(d.height, dtype=np.float32)
+    Y = np.float32(0.83) * (v - np.float32(K[3])) / np.float32(K[1])
+    inside_v = (Y >= np.float32(-0.32)) & (Y <= np.float32(-0.32) + np.float32(32 * 0.02))
+    want = np.where(inside_v[:, None] & inside_u[None, :], 830, 0)
+    assert np.array_equal(out, want) and 0 < (out > 0).sum() < out.size
+    far = orc.depth_crop(np.full((d.height, d.width), 1500, np.uint16), K, I4, vol)
+    assert not far.any()
+
+
 # ---------------------------------------------------------------- A16-A18 raycast
 def test_raycast_analytic_scene_within_half_voxel(prim_setup):
     st, d, F, Wt = prim_setup

@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["vs_img.hip", "vs_icp.hip", "vs_vol.hip", "vs_api.hip"]
+HIP_SOURCES = ["vs_img.hip", "vs_icp.hip", "vs_vol.hip", "vs_pre.hip", "vs_api.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -54,7 +54,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, lists, params, stats;
+    size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t total;
     size_t nvox;
@@ -123,6 +123,7 @@ Layout make_layout(const vs_session_desc &d) {
         L.lists = take(lists_bytes(nt, ng, L.nbricks));
     }
     L.stats = take(64);
+    L.keys = take((size_t)W * H * 4);  // depth reprojection z-buffer
     if (d.integrate_mode == VS_INTEGRATE_COLOR) {
         L.vcol = take(L.nvox * 4);
         L.rgb = take((size_t)W * H * 3);
@@ -479,8 +480,9 @@ vs_status sync_last(vs_session *s) {
     return VS_OK;
 }
 
+// A NULL frame keeps the raw buffer as it is (set by vs_set_depth or a pre-step).
 vs_status copy_depth(vs_session *s, const uint16_t *depth, int is_host, hipStream_t st) {
-    if (!depth) return fail(VS_EINVAL, "null depth");
+    if (!depth) return VS_OK;
     const size_t bytes = (size_t)s->d.width * s->d.height * 2;
     HIP_TRY(hipMemcpyAsync((void *)s->hp.raw, depth, bytes, is_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                            st));
@@ -614,6 +616,7 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
         P.origin[k] = d.origin[k];
     }
     P.vox_inv = 1.0f / d.voxel;
+    P.voxel = d.voxel;
     P.mu = d.mu;
     P.max_w = d.max_weight;
     // cube centre = origin + dim*voxel/2 (z_yooji_object_scanner.cpp:306); radius = |origin - centre|
@@ -746,8 +749,38 @@ vs_status vs_set_color(vs_session *s, const uint8_t *rgb_planar, int is_host, vs
     return VS_OK;
 }
 
-vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream) {
+vs_status vs_depth_reproject(vs_session *s, const void *src, int src_w, int src_h, const float K_src[4],
+                             const float T_dst_src[16], int src_is_float, void *dst, vs_stream stream) {
+    if (!s || !src || !K_src || !T_dst_src) return fail(VS_EINVAL, "null argument");
+    if (src_w < 1 || src_h < 1 || src_w > 8192 || src_h > 8192) return fail(VS_EINVAL, "source size %dx%d", src_w, src_h);
+    if (!dst && src_is_float) return fail(VS_EINVAL, "the session's raw buffer holds uint16 millimetres");
+    if (!(K_src[0] != 0.0f) || !(K_src[1] != 0.0f)) return fail(VS_EINVAL, "source focal length");
+    const float *Kd = s->hp.lev[0].K;
+    launch_depth_reproject(src_is_float, src, src_w, src_h, K_src, Kd, T_dst_src,
+                           (uint32_t *)(s->mem + s->L.keys), dst ? dst : (void *)s->hp.raw, s->d.width, s->d.height,
+                           pick(s, stream));
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_depth_crop_to_volume(vs_session *s, void *depth, int w, int h, const float K[4], const float *T_cg,
+                                  int is_float, vs_stream stream) {
     if (!s) return fail(VS_EINVAL, "null session");
+    if (!depth) {
+        if (is_float) return fail(VS_EINVAL, "the session's raw buffer holds uint16 millimetres");
+        depth = (void *)s->hp.raw;
+        w = s->d.width;
+        h = s->d.height;
+        K = s->hp.lev[0].K;
+    }
+    if (!K || w < 1 || h < 1 || w > 8192 || h > 8192) return fail(VS_EINVAL, "frame %dx%d / intrinsics", w, h);
+    launch_depth_crop(s->self.dP, is_float, depth, w, h, K, T_cg, pick(s, stream));
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_set_depth(vs_session *s, const uint16_t *depth_mm, int is_host, vs_stream stream) {
+    if (!s || !depth_mm) return fail(VS_EINVAL, "null argument");
     return copy_depth(s, depth_mm, is_host, pick(s, stream));
 }
 
@@ -823,7 +856,7 @@ vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs
 
 vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t *const *depths, int n, int is_host,
                         vs_stream stream) {
-    if (!ctx || !sessions || !depths || n <= 0) return fail(VS_EINVAL, "bad batch arguments");
+    if (!ctx || !sessions || n <= 0) return fail(VS_EINVAL, "bad batch arguments");
     std::vector<vs_session *> key(sessions, sessions + n);
     for (int i = 0; i < n; ++i) {
         if (!key[i] || key[i]->ctx != ctx) return fail(VS_EINVAL, "session %d not in context", i);
@@ -862,23 +895,24 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
     } else {
         B = it->second.get();
     }
-    if (is_host) {
+    if (depths && is_host) {
         for (int i = 0; i < n; ++i) {
             vs_status r = copy_depth(key[i], depths[i], 1, h);
             if (r != VS_OK) return r;
         }
-    } else {
-        // device-resident frames: one gather launch per 64 sessions
-        for (int first = 0; first < n; first += kDepthChunk) {
-            DepthChunk ch{};
-            ch.n = std::min(kDepthChunk, n - first);
-            for (int i = 0; i < ch.n; ++i) {
-                if (!depths[first + i]) return fail(VS_EINVAL, "null depth for session %d", first + i);
-                ch.src[i] = depths[first + i];
-                ch.slot[i] = first + i;
+    } else if (depths) {
+        // device-resident frames: one gather launch per 64 sessions that have one
+        DepthChunk ch{};
+        for (int i = 0; i < n; ++i) {
+            if (!depths[i]) continue;
+            ch.src[ch.n] = depths[i];
+            ch.slot[ch.n] = i;
+            if (++ch.n == kDepthChunk) {
+                launch_gather_depth(B->dP, ch, h);
+                ch.n = 0;
             }
-            launch_gather_depth(B->dP, ch, h);
         }
+        launch_gather_depth(B->dP, ch, h);
         HIP_TRY(hipGetLastError());
     }
     for (int i = 0; i < n; ++i) key[i]->last = h;
@@ -1408,7 +1442,7 @@ vs_session *vs_server_session(vs_server *v, int slot) {
 
 vs_status vs_server_step(vs_server *v, const int *slots, const uint16_t *const *depths, int n, int is_host,
                          vs_stream stream) {
-    if (!v || !slots || !depths || n <= 0) return fail(VS_EINVAL, "bad step arguments");
+    if (!v || !slots || n <= 0) return fail(VS_EINVAL, "bad step arguments");
     const int ns = (int)v->slot.size();
     if (n > ns) return fail(VS_EINVAL, "%d frames for %d slots", n, ns);
     uint32_t bits[kMaxSlots / 32] = {};
@@ -1416,7 +1450,6 @@ vs_status vs_server_step(vs_server *v, const int *slots, const uint16_t *const *
         const int i = slots[k];
         if (i < 0 || i >= ns || !v->slot[i]) return fail(VS_EINVAL, "slot %d not open", i);
         if ((bits[i >> 5] >> (i & 31)) & 1u) return fail(VS_EINVAL, "slot %d repeated in step", i);
-        if (!depths[k]) return fail(VS_EINVAL, "null depth for slot %d", i);
         bits[i >> 5] |= 1u << (i & 31);
     }
     HIP_TRY(hipSetDevice(v->ctx->device));
@@ -1431,21 +1464,23 @@ vs_status vs_server_step(vs_server *v, const int *slots, const uint16_t *const *
         HIP_TRY(hipEventRecord(v->order, v->last));
         HIP_TRY(hipStreamWaitEvent(h, v->order, 0));
     }
-    if (is_host) {
+    if (depths && is_host) {
         for (int k = 0; k < n; ++k) {
             vs_status r = copy_depth(v->slot[slots[k]], depths[k], 1, h);
             if (r != VS_OK) return r;
         }
-    } else {
-        for (int first = 0; first < n; first += kDepthChunk) {
-            DepthChunk ch{};
-            ch.n = std::min(kDepthChunk, n - first);
-            for (int k = 0; k < ch.n; ++k) {
-                ch.src[k] = depths[first + k];
-                ch.slot[k] = slots[first + k];
+    } else if (depths) {
+        DepthChunk ch{};
+        for (int k = 0; k < n; ++k) {
+            if (!depths[k]) continue;
+            ch.src[ch.n] = depths[k];
+            ch.slot[ch.n] = slots[k];
+            if (++ch.n == kDepthChunk) {
+                launch_gather_depth(v->B.dP, ch, h);
+                ch.n = 0;
             }
-            launch_gather_depth(v->B.dP, ch, h);
         }
+        launch_gather_depth(v->B.dP, ch, h);
     }
     launch_set_active(v->active, ns, bits, h);
     HIP_TRY(hipGetLastError());

@@ -77,6 +77,7 @@ struct alignas(16) SessParams {
     int bdim[3];        // bricks per axis
     float origin[3];
     float vox_inv;      // 1.0f / voxel (FP32)
+    float voxel;        // voxel edge (m)
     float mu, max_w;
     float cube_cen[3];
     float r_cube, step_scale, total_max;
@@ -106,6 +107,11 @@ struct alignas(16) SessParams {
     uint32_t *vcol;          // brick-major RGB (bits 0-7 R, 8-15 G, 16-23 B)
     const uint8_t *rgb;      // colour frame: R, G, B planes of the level-0 size
     uint8_t *mtex[kMaxLevels];  // texture maps: R, G, B planes per level
+};
+
+// a 4x4 matrix (or a few small vectors) passed to a kernel by value
+struct Mat16f {
+    float m[16];
 };
 
 struct WorkItem {

@@ -19,9 +19,6 @@ namespace vs {
 struct Mat16d {
     double m[16];
 };
-struct Mat16f {
-    float m[16];
-};
 
 // ---------------------------------------------------------------------------
 // FP64 pose algebra (one thread)

@@ -70,6 +70,11 @@ void launch_volume_from_linear(const SessParams *P, size_t nvox, int tsdf_type, 
                                const float *Wt, hipStream_t s);
 void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int weight_type, hipStream_t s);
 void launch_color_to_linear(const SessParams *P, size_t nvox, uint8_t *rgb, hipStream_t s);
+// input pre-steps (vs_pre.hip); is_float selects float metres, else uint16 mm
+void launch_depth_reproject(int is_float, const void *src, int sw, int sh, const float Ks[4], const float Kd[4],
+                            const float T[16], uint32_t *key, void *dst, int dw, int dh, hipStream_t s);
+void launch_depth_crop(const SessParams *P, int is_float, void *depth, int w, int h, const float K[4],
+                       const float *Tcg, hipStream_t s);
 // snapshots: copy the listed bricks into (unpack = 0) or out of (1) a packed buffer
 void launch_brick_pack(const SessParams *P, const uint32_t *idx, unsigned n, int value_bytes, int weight_bytes,
                        int color_bytes, uint8_t *buf, int unpack, hipStream_t s);

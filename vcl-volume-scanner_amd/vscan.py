@@ -99,7 +99,8 @@ def lib():
                      "vs_server_create", "vs_server_destroy", "vs_server_open", "vs_server_close",
                      "vs_server_step", "vs_server_usage", "vs_set_color", "vs_get_texture",
                      "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
-                     "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot"]:
+                     "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot", "vs_depth_reproject",
+                     "vs_depth_crop_to_volume"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
@@ -130,6 +131,8 @@ def lib():
         L.vs_ctx_set_timing.argtypes = [P, C.c_int]
         L.vs_ctx_stage_ms.argtypes = [P, P]
         L.vs_set_color.argtypes = [P, P, C.c_int, P]
+        L.vs_depth_reproject.argtypes = [P, P, C.c_int, C.c_int, P, P, C.c_int, P, P]
+        L.vs_depth_crop_to_volume.argtypes = [P, P, C.c_int, C.c_int, P, P, C.c_int, P]
         L.vs_session_snapshot_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
         L.vs_session_snapshot.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_size_t)]
         L.vs_session_restore.argtypes = [P, P, C.c_size_t, C.POINTER(P)]
@@ -381,8 +384,27 @@ class Session:
         check(lib().vs_raycast(self.h, level, None), "vs_raycast")
 
     def fuse(self, depth):
+        """Fuse a host uint16 frame; None fuses the frame already in the
+        session's raw buffer (vs_set_depth or a pre-step)."""
+        if depth is None:
+            check(lib().vs_fuse_frame(self.h, None, 1, None), "vs_fuse_frame")
+            return
         depth = np.ascontiguousarray(depth, np.uint16)
         check(lib().vs_fuse_frame(self.h, _ptr(depth), 1, None), "vs_fuse_frame")
+
+    def depth_reproject(self, src_ptr, w, h, K_src, T_dst_src, dst_ptr=None, is_float=False):
+        """z_tdrcc on device buffers (pointers as ints); dst None = the raw buffer."""
+        K = np.ascontiguousarray(K_src, np.float32)
+        T = np.ascontiguousarray(T_dst_src, np.float32).reshape(16)
+        check(lib().vs_depth_reproject(self.h, P(src_ptr), w, h, _ptr(K), _ptr(T), 1 if is_float else 0,
+                                       P(dst_ptr) if dst_ptr else None, None), "vs_depth_reproject")
+
+    def depth_crop(self, depth_ptr=None, w=0, h=0, K=None, T_cg=None, is_float=False):
+        """god's volume crop in place; depth_ptr None = the raw buffer."""
+        Kp = None if K is None else np.ascontiguousarray(K, np.float32)
+        Tp = None if T_cg is None else np.ascontiguousarray(T_cg, np.float32).reshape(16)
+        check(lib().vs_depth_crop_to_volume(self.h, P(depth_ptr) if depth_ptr else None, w, h, _ptr(Kp), _ptr(Tp),
+                                            1 if is_float else 0, None), "vs_depth_crop_to_volume")
 
     def sync(self):
         check(lib().vs_session_sync(self.h), "vs_session_sync")
