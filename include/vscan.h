@@ -233,6 +233,14 @@ vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long
  * the raycast's algorithmic bytes; no counterpart in the reference. */
 vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]);
 
+/* Volume carving: g_uvv_Update_Valid_Volume (z_yooji_cuda_volume_integrator.cu:718-799,
+ * LGKvVolumeIntegrator::uvv_Update_Valid_Volume :1035-1060). valid = one byte per voxel in the
+ * linear x-fastest layout (the reference's GKvVolumeBool, e.g. a visual hull), device pointer
+ * unless is_host. Every brick of the last frame's visibility mask with no valid voxel is reset to
+ * TSDF 1, weight 0, colour 0 and leaves the mask. *carved (nullable) = bricks reset
+ * (synchronous). */
+vs_status vs_volume_carve(vs_session *s, const uint8_t *valid, int is_host, unsigned *carved);
+
 /* ---- snapshots: save / restore / evict ----------------------------------
  * Replaces itsdfcd_Import_TSDF_Cube_from_Device + stsdfc_Save_TSDF_Cube
  * (z_yooji_object_scanner.cpp:2687-2745), whose .vcl files use the closed

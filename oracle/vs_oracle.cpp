@@ -1003,6 +1003,37 @@ void depth_reproject(const uint16_t *src, int sw, int sh, const float *Ks, const
         }
 }
 
+// g_uvv_Update_Valid_Volume (z_yooji_cuda_volume_integrator.cu:718-799): bricks
+// of the mask with no valid voxel are reset (TSDF 1, weight 0, colour 0) and
+// leave the mask. Returns the number reset.
+long long volume_carve(double *F, double *Wt, uint8_t *C, uint8_t *bmask, const uint8_t *valid, const Vol &V) {
+    const size_t N = (size_t)V.W * V.H * V.D;
+    long long carved = 0;
+    for (int bz = 0; bz < V.BD; ++bz)
+        for (int by = 0; by < V.BH; ++by)
+            for (int bx = 0; bx < V.BW; ++bx) {
+                const size_t b = ((size_t)bz * V.BH + by) * V.BW + bx;
+                if (!bmask[b]) continue;
+                int cnt = 0;
+                for (int z = bz * 8; z < bz * 8 + 8; ++z)
+                    for (int y = by * 8; y < by * 8 + 8; ++y)
+                        for (int x = bx * 8; x < bx * 8 + 8; ++x) cnt += valid[((size_t)z * V.H + y) * V.W + x] != 0;
+                if (cnt > 0) continue;
+                bmask[b] = 0;
+                ++carved;
+                for (int z = bz * 8; z < bz * 8 + 8; ++z)
+                    for (int y = by * 8; y < by * 8 + 8; ++y)
+                        for (int x = bx * 8; x < bx * 8 + 8; ++x) {
+                            const size_t i = ((size_t)z * V.H + y) * V.W + x;
+                            F[i] = 1.0;
+                            Wt[i] = 0.0;
+                            if (C)
+                                for (int k = 0; k < 3; ++k) C[k * N + i] = 0;
+                        }
+            }
+    return carved;
+}
+
 void depth_crop(uint16_t *mm, int W, int H, const float *K, const float *Tcg, const Vol &V) {
     const float sx = (float)V.W * V.voxel, sy = (float)V.H * V.voxel, sz = (float)V.D * V.voxel;
     for (int j = 0; j < H; ++j)
@@ -1245,6 +1276,10 @@ void or_raycast_color_d(const float *F, const float *Wt, const uint8_t *C, const
 void or_depth_reproject(const uint16_t *src, int sw, int sh, const float K_src[4], const float T_dst_src[16],
                         uint16_t *dst, int dw, int dh, const float K_dst[4]) {
     depth_reproject(src, sw, sh, K_src, T_dst_src, dst, dw, dh, K_dst);
+}
+long long or_volume_carve(double *F, double *Wt, uint8_t *C, uint8_t *brick_mask, const uint8_t *valid,
+                          const or_volume *vol) {
+    return volume_carve(F, Wt, C, brick_mask, valid, make_vol(vol));
 }
 void or_depth_crop(uint16_t *mm, int W, int H, const float K[4], const float T_cg[16], const or_volume *vol) {
     depth_crop(mm, W, H, K, T_cg, make_vol(vol));

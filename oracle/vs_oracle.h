@@ -110,6 +110,12 @@ void or_depth_reproject(const uint16_t *src, int sw, int sh, const float K_src[4
                         uint16_t *dst, int dw, int dh, const float K_dst[4]);
 void or_depth_crop(uint16_t *mm, int W, int H, const float K[4], const float T_cg[16], const or_volume *vol);
 
+/* g_uvv volume carving: bricks of brick_mask (in/out) with no valid voxel are
+ * reset (F 1, W 0, colour 0 when C is given) and leave the mask. Returns the
+ * number of bricks reset. valid: one byte per voxel, linear x-fastest. */
+long long or_volume_carve(double *F, double *Wt, uint8_t *C, uint8_t *brick_mask, const uint8_t *valid,
+                          const or_volume *vol);
+
 /* A19: the whole per-frame pipeline (so_Scan_Object). */
 typedef struct or_session or_session;
 typedef struct {

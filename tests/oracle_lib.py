@@ -253,6 +253,14 @@ def depth_reproject(src, K_src, T_dst_src, dst_shape, K_dst):
     return out
 
 
+def volume_carve(F, Wt, Cvol, bmask, valid, vol):
+    """g_uvv on FP64 F/Wt (updated in place), optional (3, N) colour and the
+    brick mask (updated in place); returns the number of bricks reset."""
+    oracle().or_volume_carve.restype = C.c_longlong
+    valid = np.ascontiguousarray(valid, np.uint8)
+    return oracle().or_volume_carve(_p(F), _p(Wt), _p(Cvol), _p(bmask), _p(valid), C.byref(vol))
+
+
 def depth_crop(mm, K, T_cg_f, vol):
     out = np.ascontiguousarray(mm, np.uint16).copy()
     H, W = out.shape

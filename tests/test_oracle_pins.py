@@ -563,6 +563,41 @@ def test_depth_crop_closed_form():
     assert not far.any()
 
 
+# ------------------------------------------------------------------- volume carving
+def test_volume_carve_closed_form():
+    """A masked brick with a single valid voxel survives untouched; masked
+    bricks with none are reset and leave the mask; unmasked bricks are never
+    touched, valid voxels or not."""
+    d = _front_camera()
+    vol = orc.volume(d)
+    N = 32 ** 3
+    rng = np.random.default_rng(3)
+    F = rng.uniform(-1, 1, N)
+    Wt = rng.integers(1, 4, N).astype(np.float64)
+    Cv = rng.integers(1, 255, (3, N)).astype(np.uint8)
+    bm = np.zeros(64, np.uint8)
+    bm[[0, 1, 2, 9]] = 1
+    valid = np.zeros((32, 32, 32), np.uint8)
+    valid[0, 0, 8 + 7] = 1  # one voxel of brick 1 (bx = 1)
+    valid[8:16, 8:16, 8:16] = 1  # all of brick (1,1,1) = 21, not in the mask
+    F0, W0_, C0 = F.copy(), Wt.copy(), Cv.copy()
+    n = orc.volume_carve(F, Wt, Cv, bm, valid, vol)
+    assert n == 3 and list(np.nonzero(bm)[0]) == [1]
+    F3, W3 = F.reshape(32, 32, 32), Wt.reshape(32, 32, 32)
+    C3 = Cv.reshape(3, 32, 32, 32)
+    for b in (0, 2, 9):
+        bz, by, bx = b // 16, (b // 4) % 4, b % 4
+        sl = (slice(bz * 8, bz * 8 + 8), slice(by * 8, by * 8 + 8), slice(bx * 8, bx * 8 + 8))
+        assert np.all(F3[sl] == 1.0) and np.all(W3[sl] == 0.0) and np.all(C3[(slice(None),) + sl] == 0)
+    keep = np.ones(N, bool).reshape(32, 32, 32)
+    for b in (0, 2, 9):
+        bz, by, bx = b // 16, (b // 4) % 4, b % 4
+        keep[bz * 8:bz * 8 + 8, by * 8:by * 8 + 8, bx * 8:bx * 8 + 8] = False
+    keep = keep.reshape(-1)
+    assert np.array_equal(F[keep], F0[keep]) and np.array_equal(Wt[keep], W0_[keep])
+    assert np.array_equal(Cv[:, keep], C0[:, keep])
+
+
 # ---------------------------------------------------------------- A16-A18 raycast
 def test_raycast_analytic_scene_within_half_voxel(prim_setup):
     st, d, F, Wt = prim_setup

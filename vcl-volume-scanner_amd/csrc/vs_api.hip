@@ -1040,6 +1040,33 @@ vs_status vs_volume_download_color(vs_session *s, uint8_t *rgb_planar) {
     return VS_OK;
 }
 
+vs_status vs_volume_carve(vs_session *s, const uint8_t *valid, int is_host, unsigned *carved) {
+    if (!s || !valid) return fail(VS_EINVAL, "null argument");
+    vs_status r = sync_last(s);
+    if (r != VS_OK) return r;
+    const size_t n = s->L.nvox;
+    char *tmp = nullptr;
+    hipError_t e = hipMalloc((void **)&tmp, 256 + (is_host ? n : 0));
+    if (e != hipSuccess) return fail(VS_ENOMEM, "carve buffer: %s", hipGetErrorString(e));
+    unsigned *cnt = (unsigned *)tmp;
+    const uint8_t *dv = valid;
+    e = hipMemset(cnt, 0, 4);
+    if (e == hipSuccess && is_host) {
+        e = hipMemcpy(tmp + 256, valid, n, hipMemcpyHostToDevice);
+        dv = (const uint8_t *)(tmp + 256);
+    }
+    if (e == hipSuccess) {
+        launch_carve(s->self.dP, s->d.tsdf_type, s->d.weight_type, dv, cnt, s->stream);
+        e = hipStreamSynchronize(s->stream);
+    }
+    unsigned c = 0;
+    if (e == hipSuccess) e = hipMemcpy(&c, cnt, 4, hipMemcpyDeviceToHost);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(VS_EHIP, "carve: %s", hipGetErrorString(e));
+    if (carved) *carved = c;
+    return VS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // snapshots
 // ---------------------------------------------------------------------------

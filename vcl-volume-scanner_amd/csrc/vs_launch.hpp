@@ -75,6 +75,9 @@ void launch_depth_reproject(int is_float, const void *src, int sw, int sh, const
                             const float T[16], uint32_t *key, void *dst, int dw, int dh, hipStream_t s);
 void launch_depth_crop(const SessParams *P, int is_float, void *depth, int w, int h, const float K[4],
                        const float *Tcg, hipStream_t s);
+// g_uvv carving: reset visible bricks with no valid voxel; *carved += their count
+void launch_carve(const SessParams *P, int tsdf_type, int weight_type, const uint8_t *valid, unsigned *carved,
+                  hipStream_t s);
 // snapshots: copy the listed bricks into (unpack = 0) or out of (1) a packed buffer
 void launch_brick_pack(const SessParams *P, const uint32_t *idx, unsigned n, int value_bytes, int weight_bytes,
                        int color_bytes, uint8_t *buf, int unpack, hipStream_t s);

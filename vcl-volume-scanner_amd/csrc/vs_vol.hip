@@ -803,6 +803,47 @@ void launch_brick_pack(const SessParams *P, const uint32_t *idx, unsigned n, int
                            unpack);
 }
 
+// g_uvv_Update_Valid_Volume (z_yooji_cuda_volume_integrator.cu:718-799): every
+// brick of the last frame's visibility mask whose voxels are all invalid in
+// `valid` (linear x-fastest, one byte per voxel) is reset to the initial
+// state (TSDF 1, weight 0, colour 0) and leaves the mask; it also leaves the
+// occupancy bits, since it holds no weight any more. One workgroup per brick,
+// 2 voxels per thread.
+template <class TV, class TW>
+__global__ __launch_bounds__(kTileThreads) void k_carve(const SessParams *__restrict__ SP, const uint8_t *valid,
+                                                        unsigned *carved) {
+    const SessParams &P = SP[0];
+    const uint32_t nb = (uint32_t)P.bdim[0] * P.bdim[1] * P.bdim[2];
+    const int t = threadIdx.x;
+    const int xl = t & 7, yl = (t >> 3) & 7, zl = t >> 6;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        if (!((P.bvis[b >> 5] >> (b & 31)) & 1u)) continue;
+        const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
+        const int x = bx * kBrick + xl, y = by * kBrick + yl;
+        bool any = false;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int z = bz * kBrick + zl + 4 * h;
+            any |= valid[((size_t)z * P.vdim[1] + y) * P.vdim[0] + x] != 0;
+        }
+        if (__syncthreads_or(any)) continue;
+        TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
+        TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int li = t + kTileThreads * h;
+            st_val(F, li, 1.0f);
+            st_w(Wt, li, 0.0f);
+            if (P.vcol) P.vcol[(size_t)b * kBrickVox + li] = 0u;
+        }
+        if (t == 0) {
+            atomicAnd(P.bvis + (b >> 5), ~(1u << (b & 31)));
+            atomicAnd(P.bocc + (b >> 5), ~(1u << (b & 31)));
+            atomicAdd(carved, 1u);
+        }
+    }
+}
+
 // brick-major RGB -> three linear x-fastest planes
 __global__ void k_color_to_linear(const SessParams *__restrict__ SP, size_t nvox, uint8_t *rgb) {
     const SessParams &P = SP[0];
@@ -968,6 +1009,14 @@ void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int wt, hip
     hipLaunchKernelGGL(k_socc_clear, dim3(16), dim3(256), 0, s, P);
     if (wt == 0) hipLaunchKernelGGL((k_occupancy<float>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
     else hipLaunchKernelGGL((k_occupancy<uint8_t>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
+}
+
+template <class TV, class TW>
+static void carve_t(const SessParams *P, const uint8_t *valid, unsigned *carved, hipStream_t s) {
+    hipLaunchKernelGGL((k_carve<TV, TW>), dim3(4096), dim3(kTileThreads), 0, s, P, valid, carved);
+}
+void launch_carve(const SessParams *P, int tt, int wt, const uint8_t *valid, unsigned *carved, hipStream_t s) {
+    VS_DISPATCH(tt, wt, carve_t, P, valid, carved, s);
 }
 
 }  // namespace vs

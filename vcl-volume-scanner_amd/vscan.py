@@ -100,7 +100,7 @@ def lib():
                      "vs_server_step", "vs_server_usage", "vs_set_color", "vs_get_texture",
                      "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
                      "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot", "vs_depth_reproject",
-                     "vs_depth_crop_to_volume"]:
+                     "vs_depth_crop_to_volume", "vs_volume_carve"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
@@ -132,6 +132,7 @@ def lib():
         L.vs_ctx_stage_ms.argtypes = [P, P]
         L.vs_set_color.argtypes = [P, P, C.c_int, P]
         L.vs_depth_reproject.argtypes = [P, P, C.c_int, C.c_int, P, P, C.c_int, P, P]
+        L.vs_volume_carve.argtypes = [P, P, C.c_int, C.POINTER(C.c_uint)]
         L.vs_depth_crop_to_volume.argtypes = [P, P, C.c_int, C.c_int, P, P, C.c_int, P]
         L.vs_session_snapshot_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
         L.vs_session_snapshot.argtypes = [P, P, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -344,6 +345,15 @@ class Session:
         buf = np.zeros(n.value, np.uint8)
         check(lib().vs_session_snapshot(self.h, _ptr(buf), n.value, C.byref(n)), "vs_session_snapshot")
         return buf[:n.value].tobytes()
+
+    def carve(self, valid):
+        """g_uvv with a host validity volume (D, H, W) of bytes; returns the
+        number of bricks reset."""
+        valid = np.ascontiguousarray(valid, np.uint8)
+        assert valid.size == self.nvox
+        n = C.c_uint()
+        check(lib().vs_volume_carve(self.h, _ptr(valid), 1, C.byref(n)), "vs_volume_carve")
+        return n.value
 
     def set_color(self, rgb):
         """rgb: (3, H, W) uint8 planes registered to the depth frame."""
