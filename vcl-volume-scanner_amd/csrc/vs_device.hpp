@@ -154,6 +154,34 @@ __device__ __forceinline__ bool roundf_ref(float v, int &out) {
     out = (v < 0.0f) ? (int)(v - 0.5f) : (int)(v + 0.5f);
     return true;
 }
+// ---------------------------------------------------------------------------
+// Address-space views. Buffer pointers held in SessParams (and any pointer
+// loaded from memory) are generic, and accesses through generic pointers
+// compile to FLAT instructions: 64-bit per-lane addresses, and each one
+// counts against both the vector-memory and the LDS/scalar wait counters, so
+// a wait for any scalar load also waits for every outstanding voxel tap.
+// Every buffer the kernels touch lives in HBM, so they read and write it
+// through these global-qualified views (global_load / global_store).
+// ---------------------------------------------------------------------------
+#define VS_GLOBAL __attribute__((address_space(1)))
+#define VS_LDS __attribute__((address_space(3)))
+typedef float vs_f4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ __forceinline__ T ldg(const T *p, size_t i) {
+    return ((const VS_GLOBAL T *)p)[i];
+}
+template <class T>
+__device__ __forceinline__ void stg(T *p, size_t i, T v) {
+    ((VS_GLOBAL T *)p)[i] = v;
+}
+__device__ __forceinline__ float4 ldg4(const float4 *p, size_t i) {
+    const vs_f4 v = ((const VS_GLOBAL vs_f4 *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg4(float4 *p, size_t i, float x, float y, float z, float w) {
+    ((VS_GLOBAL vs_f4 *)p)[i] = vs_f4{x, y, z, w};
+}
+
 __device__ __forceinline__ float depth_m(uint16_t mm) { return (float)mm * 0.001f; }
 // __float2int_rn (round half to even); non-finite or |v| >= 2^31 is out of range
 __device__ __forceinline__ bool rint_ref(float v, int &out) {
@@ -187,15 +215,19 @@ __device__ __forceinline__ int super_of_brick(const SessParams &P, int bx, int b
     return ((bz >> 2) * P.sdim[1] + (by >> 2)) * P.sdim[0] + (bx >> 2);
 }
 
-// Storage conversions for the two volume forms.
-__device__ __forceinline__ float ld_val(const float *p, size_t i) { return p[i]; }
-__device__ __forceinline__ float ld_val(const __half *p, size_t i) { return __half2float(p[i]); }
-__device__ __forceinline__ void st_val(float *p, size_t i, float v) { p[i] = v; }
-__device__ __forceinline__ void st_val(__half *p, size_t i, float v) { p[i] = __float2half_rn(v); }
-__device__ __forceinline__ float ld_w(const float *p, size_t i) { return p[i]; }
-__device__ __forceinline__ float ld_w(const uint8_t *p, size_t i) { return (float)p[i]; }
-__device__ __forceinline__ void st_w(float *p, size_t i, float v) { p[i] = v; }
-__device__ __forceinline__ void st_w(uint8_t *p, size_t i, float v) { p[i] = (uint8_t)v; }
+// Storage conversions for the two volume forms (volume planes are in HBM).
+__device__ __forceinline__ float ld_val(const float *p, size_t i) { return ldg(p, i); }
+__device__ __forceinline__ float ld_val(const __half *p, size_t i) {
+    return __half2float(__ushort_as_half(ldg((const uint16_t *)p, i)));
+}
+__device__ __forceinline__ void st_val(float *p, size_t i, float v) { stg(p, i, v); }
+__device__ __forceinline__ void st_val(__half *p, size_t i, float v) {
+    stg((uint16_t *)p, i, (uint16_t)__half_as_ushort(__float2half_rn(v)));
+}
+__device__ __forceinline__ float ld_w(const float *p, size_t i) { return ldg(p, i); }
+__device__ __forceinline__ float ld_w(const uint8_t *p, size_t i) { return (float)ldg(p, i); }
+__device__ __forceinline__ void st_w(float *p, size_t i, float v) { stg(p, i, v); }
+__device__ __forceinline__ void st_w(uint8_t *p, size_t i, float v) { stg(p, i, (uint8_t)v); }
 
 // Wave64 sum via cross-lane shuffles (ds_bpermute: an LDS round trip per step).
 __device__ __forceinline__ float wave_sum(float v) {

@@ -133,7 +133,7 @@ struct Quad {
     float4 a, b, c, d;
 };
 __device__ __forceinline__ Quad load_quad(const float4 *m, size_t i0, size_t di, size_t dj) {
-    return {m[i0], m[i0 + di], m[i0 + dj], m[i0 + di + dj]};
+    return {ldg4(m, i0), ldg4(m, i0 + di), ldg4(m, i0 + dj), ldg4(m, i0 + di + dj)};
 }
 __device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &out) {
     f3 t1{0, 0, 0}, t2{0, 0, 0}, t3{0, 0, 0}, t4{0, 0, 0};
@@ -160,7 +160,7 @@ __device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, i
                                             float &b) {
     const LevelDesc &L = P.lev[l];
     const int W = L.W, H = L.H;
-    const float td1 = P.filt[l][(size_t)y * W + x];
+    const float td1 = ldg(P.filt[l], (size_t)y * W + x);
     if (td1 < 1.0e-6f) return false;
     const f3 p = transform(T21, back_project((float)x, (float)y, L.K, td1));
     float u, v;
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
         for (int k = 0; k < kIcpVals; ++k) acc[k] += v[k];
     }
     block_terms(acc, red64);
-    if (threadIdx.x < kIcpVals) P.icp_part[(size_t)wi.tile * kIcpVals + threadIdx.x] = red64[threadIdx.x];
+    if (threadIdx.x < kIcpVals) stg(P.icp_part, (size_t)wi.tile * kIcpVals + threadIdx.x, red64[threadIdx.x]);
 }
 
 // Second half: one workgroup per session sums the group partials in FP64 in
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
     for (int t = threadIdx.x; t < ngroups; t += kTileThreads) {
         const double *pp = P.icp_part + (size_t)t * kIcpVals;
 #pragma unroll
-        for (int k = 0; k < kIcpVals; ++k) acc[k] += pp[k];
+        for (int k = 0; k < kIcpVals; ++k) acc[k] += ldg(pp, k);
     }
     __shared__ double wred[kTileThreads / 64][kIcpVals];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
         int ly = i / TW, lx = i - ly * TW;
         int gx = min(max(x0 + lx - kHalo, 0), W - 1);
         int gy = min(max(y0 + ly - kHalo, 0), H - 1);
-        tile[ly][lx] = depth_m(raw[(size_t)gy * W + gx]);
+        tile[ly][lx] = depth_m(ldg(raw, (size_t)gy * W + gx));
     }
     __syncthreads();
     const int lx = threadIdx.x % kTileW, ly = threadIdx.x / kTileW;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
             s += w;
         }
     }
-    P.filt[0][(size_t)y * W + x] = zero ? 0.0f : c + t / s;
+    stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
 }
 
 // One pyramid step, level l-1 -> l: the separable [1 4 6 4 1]^2/256 blur of
@@ -79,12 +79,13 @@ __global__ __launch_bounds__(kTileThreads) void k_pyr_down(const SessParams *__r
         float h[5];
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            const float *row = in + (size_t)(j + r - 2) * Wi;
-            h[r] = 1.0f * row[i - 2] + 1.0f * row[i + 2] + 4.0f * row[i - 1] + 4.0f * row[i + 1] + 6.0f * row[i];
+            const size_t row = (size_t)(j + r - 2) * Wi;
+            h[r] = 1.0f * ldg(in, row + i - 2) + 1.0f * ldg(in, row + i + 2) + 4.0f * ldg(in, row + i - 1) +
+                   4.0f * ldg(in, row + i + 1) + 6.0f * ldg(in, row + i);
         }
         v = (1.0f * h[0] + 1.0f * h[4] + 4.0f * h[1] + 4.0f * h[3] + 6.0f * h[2]) / (16.0f * 16.0f);
     }
-    P.filt[l][(size_t)y * L.W + x] = v;
+    stg(P.filt[l], (size_t)y * L.W + x, v);
 }
 
 // Vertex and normal maps of pyramid level l (AoS float3, -100 invalid).
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(kTileThreads) void k_vnmap(const SessParams *__rest
     const float *D = P.filt[l];
     const float *K = L.K;
     auto vert = [&](int xx, int yy, f3 &v) -> bool {
-        float d = D[(size_t)yy * W + xx];
+        float d = ldg(D, (size_t)yy * W + xx);
         if (d <= 0.0f) return false;
         v = {d * ((float)xx - K[2]) / K[0], d * ((float)yy - K[3]) / K[1], d};
         return true;

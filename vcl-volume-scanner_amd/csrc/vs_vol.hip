@@ -32,7 +32,7 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
     if (!P.st->fuse_ok) return;
     const uint32_t nb = (uint32_t)P.bdim[0] * P.bdim[1] * P.bdim[2];
     const uint32_t nw = (nb + 31) / 32;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) P.bvis[w] = 0u;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) stg(P.bvis, w, 0u);
 }
 
 // Marks the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
     __syncthreads();
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
-    const float td = (x < L.W && y < L.H) ? depth_m(P.raw[(size_t)y * L.W + x]) : 0.0f;
+    const float td = (x < L.W && y < L.H) ? depth_m(ldg(P.raw, (size_t)y * L.W + x)) : 0.0f;
     if (td > 0.0f) {
         const float *Tcg = st->T_cg_f;
         const f3 p = transform(Tcg, back_project((float)x, (float)y, L.K, td));
@@ -121,18 +121,18 @@ __device__ __forceinline__ bool gii(const uint8_t *plane, int W, int H, float xf
     float d2 = 0.0f, d3 = 0.0f, d4 = 0.0f;
     const int x = (int)xf, y = (int)yf;
     const float rx = xf - (float)x, ry = yf - (float)y;
-    const float d1 = (float)plane[y * W + x];
+    const float d1 = (float)ldg(plane, (size_t)y * W + x);
     if (d1 <= 0.0f) return false;
     if (rx > 0.0f) {
-        d2 = (float)plane[y * W + x + 1];
+        d2 = (float)ldg(plane, (size_t)y * W + x + 1);
         if (d2 <= 0.0f) return false;
     }
     if (ry > 0.0f) {
-        d3 = (float)plane[(y + 1) * W + x];
+        d3 = (float)ldg(plane, (size_t)(y + 1) * W + x);
         if (d3 <= 0.0f) return false;
     }
     if (rx > 0.0f && ry > 0.0f) {
-        d4 = (float)plane[(y + 1) * W + x + 1];
+        d4 = (float)ldg(plane, (size_t)(y + 1) * W + x + 1);
         if (d4 <= 0.0f) return false;
     }
     out = (1.0f - ry) * ((1.0f - rx) * d1 + rx * d2) + ry * ((1.0f - rx) * d3 + rx * d4);
@@ -155,7 +155,7 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     int x, y;
     if (!rint_ref(u, x) || !rint_ref(v, y)) return false;
     if (x < 0 || x > L.W - 1 || y < 0 || y > L.H - 1) return false;
-    const float dm = depth_m(P.raw[(size_t)y * L.W + x]);
+    const float dm = depth_m(ldg(P.raw, (size_t)y * L.W + x));
     if (!(dm > 0.0f)) return false;
     const float r1x = 1.0f * ((float)x - L.K[2]) / L.K[0], r1y = 1.0f * ((float)y - L.K[3]) / L.K[1];
     const float lambda_inv = 1.0f / sqrtf(r1x * r1x + r1y * r1y + 1.0f);
@@ -169,7 +169,7 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     const float tsd = fminf(1.0f, sdf / P.mu);
     st_val(F, li, (w * ld_val(F, li) + tsd) * inw);
     st_w(Wt, li, nw);
-    uint32_t c = col[li];
+    uint32_t c = ldg(col, li);
     const size_t plane = (size_t)L.W * L.H;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -179,7 +179,7 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
         const uint32_t q = (uint32_t)(uint8_t)fmaxf(0.0f, fminf(255.0f, val));
         c = (c & ~(255u << (8 * k))) | (q << (8 * k));
     }
-    col[li] = c;
+    stg(col, li, c);
     return true;
 }
 
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             int iu, iv;
             if (!roundf_ref(u, iu) || !roundf_ref(v, iv)) continue;
             if (iu < 0 || iu > L.W - 1 || iv < 0 || iv > L.H - 1) continue;
-            const float dm = depth_m(P.raw[(size_t)iv * L.W + iu]);
+            const float dm = depth_m(ldg(P.raw, (size_t)iv * L.W + iu));
             if (!(dm > 0.0f)) continue;
             const float td = dm - p.z;
             if (td < -P.mu) continue;
@@ -268,7 +268,11 @@ struct Vol {
     const SessParams &P;
     const TV *F;
     const TW *Wt;
-    const uint32_t *sbits;  // superbrick bits (LDS copy when it fits)
+    // superbrick bits: the workgroup's LDS copy of kSoccLdsWords words. A
+    // session whose bitmap does not fit gets a copy of all ones (every
+    // superbrick counts as occupied and only the brick bitmap decides); the
+    // index is masked to the copy's size either way.
+    const VS_LDS uint32_t *sbits;
     // occupancy of the last brick looked up (a ray's consecutive taps mostly
     // stay in one brick; both bitmaps are read-only during the launch), in
     // one register: brick index in bits 0-29 (< 2^24 bricks), bit 30 = its
@@ -280,11 +284,11 @@ struct Vol {
     __device__ __forceinline__ bool occupied(int x, int y, int z) const {
         const unsigned b = (unsigned)brick_of(P, x, y, z);
         if (b != (cbf & kIdx)) {
-            const int sb = super_of_brick(P, x >> 3, y >> 3, z >> 3);
             unsigned f = b;
-            if ((sbits[sb >> 5] >> (sb & 31)) & 1u) {
+            const unsigned sb = (unsigned)super_of_brick(P, x >> 3, y >> 3, z >> 3);
+            if ((sbits[(sb >> 5) & (kSoccLdsWords - 1)] >> (sb & 31)) & 1u) {
                 f |= kSuperOcc;
-                if ((P.bocc[b >> 5] >> (b & 31)) & 1u) f |= kBrickOcc;
+                if ((ldg(P.bocc, b >> 5) >> (b & 31)) & 1u) f |= kBrickOcc;
             }
             cbf = f;
         }
@@ -598,7 +602,7 @@ __device__ __forceinline__ uint32_t rgb_interpolated(const Vol<TV, TW> &V, const
     for (int k = 0; k < 8; ++k) {
         const size_t idx = vox_index(V.P, ox + (k & 1), oy + ((k >> 1) & 1), oz + (k >> 2));
         ok &= ld_w(V.Wt, idx) > 0.0f;
-        c[k] = col[idx];
+        c[k] = ldg(col, idx);
     }
     if (!ok) return 0u;
     uint32_t out = 0u;
@@ -618,9 +622,9 @@ __device__ __forceinline__ uint32_t rgb_interpolated(const Vol<TV, TW> &V, const
 
 __device__ __forceinline__ void write_texture(const SessParams &P, int level, size_t t, uint32_t rgb) {
     const size_t plane = (size_t)P.lev[level].W * P.lev[level].H;
-    P.mtex[level][t] = (uint8_t)(rgb & 255u);
-    P.mtex[level][plane + t] = (uint8_t)((rgb >> 8) & 255u);
-    P.mtex[level][2 * plane + t] = (uint8_t)((rgb >> 16) & 255u);
+    stg(P.mtex[level], t, (uint8_t)(rgb & 255u));
+    stg(P.mtex[level], plane + t, (uint8_t)((rgb >> 8) & 255u));
+    stg(P.mtex[level], 2 * plane + t, (uint8_t)((rgb >> 16) & 255u));
 }
 
 // The raycast runs as two launches per level. k_raycast marches every ray
@@ -651,11 +655,12 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
     const SessState *st = P.st;
     const LevelDesc &L = P.lev[level];
     if (!st->fuse_ok || level >= P.levels || tile_out(L, wi.tile)) return;
-    const bool lds_bits = P.socc_words <= kSoccLdsWords;
-    if (lds_bits) {
-        for (int i = threadIdx.x; i < P.socc_words; i += kTileThreads) s_socc[i] = P.socc[i];
-        __syncthreads();
+    if (P.socc_words <= kSoccLdsWords) {
+        for (int i = threadIdx.x; i < P.socc_words; i += kTileThreads) s_socc[i] = ldg(P.socc, i);
+    } else {
+        for (int i = threadIdx.x; i < kSoccLdsWords; i += kTileThreads) s_socc[i] = 0xFFFFFFFFu;
     }
+    __syncthreads();
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     if (x >= L.W || y >= L.H) return;
@@ -670,17 +675,17 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
     C.step_scale = P.step_scale;
     C.total_max = P.total_max;
     C.max_iter = P.max_iter;
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, lds_bits ? s_socc : P.socc};
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint32_t *)s_socc};
     unsigned n_near = 0, n_tri = 0;
     f3 p;
     const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
-        P.mvtx[level][t] = make_float4(p.x, p.y, p.z, 1.0f);
+        stg4(P.mvtx[level], t, p.x, p.y, p.z, 1.0f);
     } else {
-        P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
-        P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
-        P.mimg[level][t] = 0;
+        stg4(P.mvtx[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
+        stg4(P.mnrm[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
+        stg(P.mimg[level], t, (uint8_t)0);
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
     if (STATS) {
@@ -702,9 +707,9 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     if (x >= L.W || y >= L.H) return;
     const size_t t = (size_t)y * L.W + x;
-    const float4 h = P.mvtx[level][t];
+    const float4 h = ldg4(P.mvtx[level], t);
     if (h.w != 1.0f) return;  // a miss: final already
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, P.socc};
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, nullptr};
     const f3 p = {h.x, h.y, h.z};
     f3 n;
     if (surface_normal(V, p, n)) {
@@ -712,11 +717,11 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
         const f3 pc = transform(T, p);
         const f3 nc = {T[0] * n.x + T[1] * n.y + T[2] * n.z, T[4] * n.x + T[5] * n.y + T[6] * n.z,
                        T[8] * n.x + T[9] * n.y + T[10] * n.z};
-        P.mvtx[level][t] = make_float4(pc.x, pc.y, pc.z, 0.0f);
-        P.mnrm[level][t] = make_float4(nc.x, nc.y, nc.z, 0.0f);
+        stg4(P.mvtx[level], t, pc.x, pc.y, pc.z, 0.0f);
+        stg4(P.mnrm[level], t, nc.x, nc.y, nc.z, 0.0f);
         // light (0,0,-1): n.l = 0*nx + 0*ny - nz (z_yooji_object_scanner.cpp:1313)
         const float sa = nc.x * 0.0f + nc.y * 0.0f + nc.z * -1.0f;
-        P.mimg[level][t] = (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f)));
+        stg(P.mimg[level], t, (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f))));
         if (P.mtex[level]) {
             f3 pv;
             world_to_voxel(P, p, pv);
@@ -724,9 +729,9 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
         }
         if (STATS) atomicAdd(stats + 2, 1ull);
     } else {
-        P.mvtx[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
-        P.mnrm[level][t] = make_float4(-100.0f, -100.0f, -100.0f, 0.0f);
-        P.mimg[level][t] = 0;
+        stg4(P.mvtx[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
+        stg4(P.mnrm[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
+        stg(P.mimg[level], t, (uint8_t)0);
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
 }
@@ -742,18 +747,18 @@ __global__ void k_volume_init(const SessParams *__restrict__ SP, size_t nvox, ui
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
         st_val(F, i, 1.0f);
         st_w(Wt, i, 0.0f);
-        if (P.vcol) P.vcol[i] = 0u;
+        if (P.vcol) stg(P.vcol, i, 0u);
     }
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (size_t)gridDim.x * blockDim.x) {
-        P.bvis[w] = 0u;
-        P.bocc[w] = 0u;
-        if (w < (size_t)P.socc_words) P.socc[w] = 0u;
+        stg(P.bvis, w, 0u);
+        stg(P.bocc, w, 0u);
+        if (w < (size_t)P.socc_words) stg(P.socc, w, 0u);
     }
 }
 
 __global__ void k_socc_clear(const SessParams *__restrict__ SP) {
     const SessParams &P = SP[0];
-    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < P.socc_words; w += gridDim.x * blockDim.x) P.socc[w] = 0u;
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < P.socc_words; w += gridDim.x * blockDim.x) stg(P.socc, (size_t)w, 0u);
 }
 
 // brick-major -> linear x-fastest, converted to float
@@ -817,14 +822,14 @@ __global__ __launch_bounds__(kTileThreads) void k_carve(const SessParams *__rest
     const int t = threadIdx.x;
     const int xl = t & 7, yl = (t >> 3) & 7, zl = t >> 6;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-        if (!((P.bvis[b >> 5] >> (b & 31)) & 1u)) continue;
+        if (!((ldg(P.bvis, b >> 5) >> (b & 31)) & 1u)) continue;
         const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
         const int x = bx * kBrick + xl, y = by * kBrick + yl;
         bool any = false;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int z = bz * kBrick + zl + 4 * h;
-            any |= valid[((size_t)z * P.vdim[1] + y) * P.vdim[0] + x] != 0;
+            any |= ldg(valid, ((size_t)z * P.vdim[1] + y) * P.vdim[0] + x) != 0;
         }
         if (__syncthreads_or(any)) continue;
         TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
@@ -834,7 +839,7 @@ __global__ __launch_bounds__(kTileThreads) void k_carve(const SessParams *__rest
             const int li = t + kTileThreads * h;
             st_val(F, li, 1.0f);
             st_w(Wt, li, 0.0f);
-            if (P.vcol) P.vcol[(size_t)b * kBrickVox + li] = 0u;
+            if (P.vcol) stg(P.vcol, (size_t)b * kBrickVox + li, 0u);
         }
         if (t == 0) {
             atomicAnd(P.bvis + (b >> 5), ~(1u << (b & 31)));
@@ -850,7 +855,7 @@ __global__ void k_color_to_linear(const SessParams *__restrict__ SP, size_t nvox
     const size_t WH = (size_t)P.vdim[0] * P.vdim[1];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
         const int z = (int)(i / WH), y = (int)((i / P.vdim[0]) % P.vdim[1]), x = (int)(i % P.vdim[0]);
-        const uint32_t c = P.vcol[vox_index(P, x, y, z)];
+        const uint32_t c = ldg(P.vcol, vox_index(P, x, y, z));
         rgb[i] = (uint8_t)(c & 255u);
         rgb[nvox + i] = (uint8_t)((c >> 8) & 255u);
         rgb[2 * nvox + i] = (uint8_t)((c >> 16) & 255u);
