@@ -54,7 +54,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, socc, icp_part, diag_part, diag_mask, lists, params, stats, keys;
+    size_t tsdf, weight, bvis, bocc, bband, sband, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t total;
     size_t nvox;
@@ -110,7 +110,8 @@ Layout make_layout(const vs_session_desc &d) {
     L.weight = take(L.nvox * sw);
     L.bvis = take((size_t)L.nwords * 4);
     L.bocc = take((size_t)L.nwords * 4);
-    L.socc = take((size_t)L.nswords * 4);
+    L.bband = take((size_t)L.nwords * 4);
+    L.sband = take((size_t)L.nswords * 4);
     L.icp_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_mask = take((size_t)W * H);
@@ -646,9 +647,10 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.weight = m + L.weight;
     P.bvis = (uint32_t *)(m + L.bvis);
     P.bocc = (uint32_t *)(m + L.bocc);
-    P.socc = (uint32_t *)(m + L.socc);
+    P.bband = (uint32_t *)(m + L.bband);
+    P.sband = (uint32_t *)(m + L.sband);
     for (int k = 0; k < 3; ++k) P.sdim[k] = L.sdim[k];
-    P.socc_words = (int)L.nswords;
+    P.sband_words = (int)L.nswords;
     P.icp_part = (double *)(m + L.icp_part);
     P.st = (SessState *)(m + L.state);
     P.tsdf_type = d.tsdf_type;
@@ -1192,7 +1194,7 @@ vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_sess
                 launch_brick_pack(s->self.dP, (const uint32_t *)dev, (unsigned)n, (int)h.value_bytes,
                                   (int)h.weight_bytes, (int)h.color_bytes, (uint8_t *)dev + align_up(n * 4), 1,
                                   s->stream);
-                launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.weight_type, s->stream);
+                launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->stream);
                 e = hipStreamSynchronize(s->stream);
             }
             hipFree(dev);
@@ -1277,7 +1279,7 @@ vs_status vs_volume_upload(vs_session *s, const float *tsdf, const float *weight
     if (e == hipSuccess) e = hipMemcpy(tmp + N, weight, N * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         launch_volume_from_linear(s->self.dP, N, s->d.tsdf_type, s->d.weight_type, tmp, tmp + N, s->stream);
-        launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.weight_type, s->stream);
+        launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->stream);
         e = hipStreamSynchronize(s->stream);
     }
     hipFree(tmp);

@@ -27,7 +27,7 @@ constexpr int kIcpVals = 29;
 constexpr int kIcpGroup = 4;
 __host__ __device__ constexpr int icp_groups(int tiles) { return (tiles + kIcpGroup - 1) / kIcpGroup; }        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
 constexpr int kSuper = 32;          // superbrick edge (voxels) = 4 bricks
-constexpr int kSoccLdsWords = 1024; // superbrick bits a raycast workgroup keeps in LDS (up to 1024^3 voxels)
+constexpr int kSbandLdsWords = 1024; // superbrick band bits a raycast workgroup keeps in LDS (up to 1024^3 voxels)
 
 struct LevelDesc {
     int W, H;
@@ -95,9 +95,14 @@ struct alignas(16) SessParams {
     void *weight;            // brick-major weights
     uint32_t *bvis;          // visible-brick bits of the current frame
     uint32_t *bocc;          // bricks holding any weight > 0 (monotone)
-    uint32_t *socc;          // superbricks (4^3 bricks = 32^3 voxels) holding an occupied brick
+    // Band bits (set-only, so a superset): a brick whose bit is clear holds
+    // only voxels of weight 0 or value exactly 1.0, which the raycast's march
+    // steps through like empty space; a superbrick (4^3 bricks = 32^3 voxels)
+    // whose bit is clear holds no band brick.
+    uint32_t *bband;
+    uint32_t *sband;
     int sdim[3];             // superbricks per axis
-    int socc_words;          // 32-bit words of socc
+    int sband_words;         // 32-bit words of sband
     double *icp_part;        // ICP group partials (kIcpVals per group of level-0 tiles)
     SessState *st;
     int tsdf_type, weight_type;

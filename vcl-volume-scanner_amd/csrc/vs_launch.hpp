@@ -68,7 +68,7 @@ void launch_volume_to_linear(const SessParams *P, size_t nvox, int tsdf_type, in
                              hipStream_t s);
 void launch_volume_from_linear(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, const float *F,
                                const float *Wt, hipStream_t s);
-void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int weight_type, hipStream_t s);
+void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tsdf_type, int weight_type, hipStream_t s);
 void launch_color_to_linear(const SessParams *P, size_t nvox, uint8_t *rgb, hipStream_t s);
 // input pre-steps (vs_pre.hip); is_float selects float metres, else uint16 mm
 void launch_depth_reproject(int is_float, const void *src, int sw, int sh, const float Ks[4], const float Kd[4],

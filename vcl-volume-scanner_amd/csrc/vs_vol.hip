@@ -9,8 +9,10 @@
 // stencils of the raycast mostly stay inside one or two bricks.
 //
 // Per-session bitmaps: `bvis` (bricks touched by the current frame, rebuilt
-// each frame) and `bocc` (bricks holding any voxel of weight > 0; weights never
-// decrease, so the bit is set once and kept).
+// each frame), `bocc` (bricks holding any voxel of weight > 0; weights never
+// decrease, so the bit is set once and kept) and the band bits `bband` /
+// `sband` (bricks / superbricks that may hold a voxel of weight > 0 and a value
+// other than 1.0; set when an update writes such a value, and kept).
 #include "vs_device.hpp"
 #include "vs_launch.hpp"
 
@@ -147,7 +149,7 @@ __device__ __forceinline__ bool gii(const uint8_t *plane, int W, int H, float xf
 // when the voxel was updated.
 template <class TV, class TW>
 __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const float *T, const float *Tcg, f3 g,
-                                                      TV *F, TW *Wt, uint32_t *col, int li) {
+                                                      TV *F, TW *Wt, uint32_t *col, int li, bool &band) {
     const LevelDesc &L = P.lev[0];
     const f3 p = transform(T, g);
     float u, v;
@@ -167,7 +169,9 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     if (w >= P.max_w) return false;  // frozen voxel
     const float nw = w + 1.0f, inw = 1.0f / nw;
     const float tsd = fminf(1.0f, sdf / P.mu);
-    st_val(F, li, (w * ld_val(F, li) + tsd) * inw);
+    const float nv = (w * ld_val(F, li) + tsd) * inw;
+    band |= nv != 1.0f;
+    st_val(F, li, nv);
     st_w(Wt, li, nw);
     uint32_t c = ldg(col, li);
     const size_t plane = (size_t)L.W * L.H;
@@ -204,13 +208,14 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
         const float gx = ((float)(bx * kBrick + xl) + 0.5f) / P.vox_inv + P.origin[0];
         const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
         int nupd = 0;
+        bool band = false;  // an updated voxel's new value is not exactly 1.0
         if (P.integ_mode == 1) {
             uint32_t *col = P.vcol + (size_t)b * kBrickVox;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int zc = bz * kBrick + zl + 4 * h;
                 const float gz = ((float)zc + 0.5f) / P.vox_inv + P.origin[2];
-                nupd += integrate_voxel_color(P, T, st->T_cg_f, {gx, gy, gz}, F, Wt, col, t + kTileThreads * h);
+                nupd += integrate_voxel_color(P, T, st->T_cg_f, {gx, gy, gz}, F, Wt, col, t + kTileThreads * h, band);
             }
         } else {
 #pragma unroll
@@ -231,20 +236,28 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             const float tsd = fminf(1.0f, __fdividef(td, P.mu));
             const float w = ld_w(Wt, li), f = ld_val(F, li);
             const float nw = w + 1.0f;
-            st_val(F, li, __fdividef(fmaf(w, f, tsd), nw));
+            const float nv = __fdividef(fmaf(w, f, tsd), nw);
+            band |= nv != 1.0f;
+            st_val(F, li, nv);
             st_w(Wt, li, fminf(nw, P.max_w));
             ++nupd;
         }
         }
         const int c1 = __syncthreads_count(nupd >= 1), c2 = __syncthreads_count(nupd >= 2);
+        const int cb = __syncthreads_or(band);
         if (t == 0 && c1 > 0) {
-            // occupancy bits only ever get set: skip the device atomic when a
-            // plain read already shows the bit (a stale 0 costs one atomic)
+            // occupancy and band bits only ever get set: skip the device
+            // atomic when a plain read already shows the bit (a stale 0 costs
+            // one atomic)
             const uint32_t bb = 1u << (b & 31);
             if (!(__atomic_load_n(P.bocc + (b >> 5), __ATOMIC_RELAXED) & bb)) atomicOr(P.bocc + (b >> 5), bb);
-            const int sb = super_of_brick(P, bx, by, bz);
-            const uint32_t sbb = 1u << (sb & 31);
-            if (!(__atomic_load_n(P.socc + (sb >> 5), __ATOMIC_RELAXED) & sbb)) atomicOr(P.socc + (sb >> 5), sbb);
+            if (cb) {
+                if (!(__atomic_load_n(P.bband + (b >> 5), __ATOMIC_RELAXED) & bb)) atomicOr(P.bband + (b >> 5), bb);
+                const int sb = super_of_brick(P, bx, by, bz);
+                const uint32_t sbb = 1u << (sb & 31);
+                if (!(__atomic_load_n(P.sband + (sb >> 5), __ATOMIC_RELAXED) & sbb))
+                    atomicOr(P.sband + (sb >> 5), sbb);
+            }
             if (it.x != run_sess) {
                 if (run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
                 run_sess = it.x;
@@ -259,40 +272,42 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
 // ---------------------------------------------------------------------------
 // A16-A18: raycast (d_crtc_Cast_Ray_on_TSDF_Cube_NEW + d_csnt + g_rmi,
 // tsdf_renderer.cu:68-153, 288-635, 1069-1209). FP32 march with the
-// reference's step rules; taps in bricks whose occupancy bit is clear read as
-// "no data" (1.0) without touching the volume, which is exactly what the
-// reference's weight test returns there.
+// reference's step rules. After its first tap a ray's march depends only on
+// the values its taps read, not on their validity, and every voxel of a
+// brick whose band bit is clear reads 1.0 (weight 0 reads as "no data" = 1.0,
+// the others hold exactly 1.0): the march reads such bricks as 1.0 without
+// touching the volume, and skips the steps that stay inside them.
 // ---------------------------------------------------------------------------
 template <class TV, class TW>
 struct Vol {
     const SessParams &P;
     const TV *F;
     const TW *Wt;
-    // superbrick bits: the workgroup's LDS copy of kSoccLdsWords words. A
-    // session whose bitmap does not fit gets a copy of all ones (every
-    // superbrick counts as occupied and only the brick bitmap decides); the
+    // superbrick band bits: the workgroup's LDS copy of kSbandLdsWords words.
+    // A session whose bitmap does not fit gets a copy of all ones (every
+    // superbrick counts as a band one and only the brick bits decide); the
     // index is masked to the copy's size either way.
     const VS_LDS uint32_t *sbits;
-    // occupancy of the last brick looked up (a ray's consecutive taps mostly
-    // stay in one brick; both bitmaps are read-only during the launch), in
-    // one register: brick index in bits 0-29 (< 2^24 bricks), bit 30 = its
-    // superbrick is occupied, bit 31 = the brick is occupied
-    static constexpr unsigned kIdx = 0x3FFFFFFFu, kSuperOcc = 1u << 30, kBrickOcc = 1u << 31;
+    // band bits of the last brick looked up (a ray's consecutive taps mostly
+    // stay in one brick; the bitmaps are read-only during the launch), in one
+    // register: brick index in bits 0-29 (< 2^24 bricks), bit 30 = its
+    // superbrick is a band one, bit 31 = the brick is a band one
+    static constexpr unsigned kIdx = 0x3FFFFFFFu, kSuperBand = 1u << 30, kBrickBand = 1u << 31;
     mutable unsigned cbf = kIdx;
     __device__ __forceinline__ int cb() const { return (int)(cbf & kIdx); }
-    __device__ __forceinline__ bool super_occupied() const { return cbf & kSuperOcc; }
-    __device__ __forceinline__ bool occupied(int x, int y, int z) const {
+    __device__ __forceinline__ bool super_band() const { return cbf & kSuperBand; }
+    __device__ __forceinline__ bool band(int x, int y, int z) const {
         const unsigned b = (unsigned)brick_of(P, x, y, z);
         if (b != (cbf & kIdx)) {
             unsigned f = b;
             const unsigned sb = (unsigned)super_of_brick(P, x >> 3, y >> 3, z >> 3);
-            if ((sbits[(sb >> 5) & (kSoccLdsWords - 1)] >> (sb & 31)) & 1u) {
-                f |= kSuperOcc;
-                if ((ldg(P.bocc, b >> 5) >> (b & 31)) & 1u) f |= kBrickOcc;
+            if ((sbits[(sb >> 5) & (kSbandLdsWords - 1)] >> (sb & 31)) & 1u) {
+                f |= kSuperBand;
+                if ((ldg(P.bband, b >> 5) >> (b & 31)) & 1u) f |= kBrickBand;
             }
             cbf = f;
         }
-        return cbf & kBrickOcc;
+        return cbf & kBrickBand;
     }
     __device__ __forceinline__ bool in_grid(int x, int y, int z) const {
         return x >= 0 && x < P.vdim[0] && y >= 0 && y < P.vdim[1] && z >= 0 && z < P.vdim[2];
@@ -306,21 +321,24 @@ struct Vol {
         out = ok ? f : 1.0f;
         return ok;
     }
-    // One tap with the occupancy skip (the march's nearest taps).
-    __device__ __forceinline__ bool tap(int x, int y, int z, float &out) const {
+    // d_gtvu, exactly: nearest voxel by ROUNDF; true when it has weight.
+    __device__ __forceinline__ bool nearest_valid(f3 v, float &out) const {
+        int x, y, z;
         out = 1.0f;
+        if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z)) return false;
         if (!in_grid(x, y, z)) return false;
-        if (!occupied(x, y, z)) return false;
-        return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out);
+        return load(vox_index(P, x, y, z), out);
     }
-    // d_gtvu: nearest voxel by ROUNDF. Returns 1 (data), 0 (no data) or -1
-    // when the rounded position lies outside the grid on an axis along which
-    // the ray moves away or not at all: every later position of a ray that
-    // keeps stepping forward is then outside as well. For "no data" it also
-    // reports what the lookup learnt about the neighbourhood: size > 0 when
-    // the voxel box [lo, lo + size)^3 (an empty superbrick or brick) holds no
-    // data at all, size = -1 when the position is outside the grid (moving
-    // towards it), size = 0 otherwise.
+    // d_gtvu's value as the march uses it: the value of the nearest voxel by
+    // ROUNDF, 1.0 in a brick without band bit. Returns 1 (a band brick's
+    // value), 0 (1.0 read without a load) or -1 when the rounded position
+    // lies outside the grid on an axis along which the ray moves away or not
+    // at all: every later position of a ray that keeps stepping forward is
+    // then outside as well. For 0 it also reports what the lookup learnt
+    // about the neighbourhood: size > 0 when every voxel of the box
+    // [lo, lo + size)^3 (a superbrick or brick without band bit) reads 1.0,
+    // size = -1 when the position is outside the grid (moving towards it),
+    // size = 0 otherwise.
     __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out, int3 &lo, int &size) const {
         int x, y, z;
         out = 1.0f;
@@ -334,18 +352,13 @@ struct Vol {
             size = -1;
             return 0;
         }
-        if (!occupied(x, y, z)) {
-            const int sh = super_occupied() ? 3 : 5;  // empty brick, or a whole empty superbrick
+        if (!band(x, y, z)) {
+            const int sh = super_band() ? 3 : 5;  // a brick, or a whole superbrick, without band bit
             lo = make_int3((x >> sh) << sh, (y >> sh) << sh, (z >> sh) << sh);
             size = 1 << sh;
             return 0;
         }
         return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
-    }
-    __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out) const {
-        int3 lo;
-        int size;
-        return nearest(v, rd, out, lo, size);
     }
     // d_gtvi: trilinear with toInt = truncation; any invalid tap -> false,
     // 1.0. The 2x2x2 stencil is range-checked as a whole and its 8 values and
@@ -376,14 +389,17 @@ struct RayConst {
     int max_iter;
 };
 
-// Exact fast-forward of the march through a region that holds no data.
-// While every tap reads "no data" (1.0) the march repeats the same step S:
+// Exact fast-forward of the march through a region whose voxels all read
+// 1.0. While every tap reads 1.0 the march repeats the same step S:
 // position += S*rd (the same FP32 product every time), total += S, iter += 1,
 // and nothing else changes. These return how many further steps certainly
 // keep the ROUNDF position inside the region: the bounds carry a 0.25-voxel
-// margin, far above the rounding drift of at most 64 FP32 additions of
-// positions below 2^13 voxels (< 0.02 voxel), so each skipped step's tap would
-// have read "no data" too.
+// margin, far above both the rounding drift of at most 64 FP32 additions of
+// positions below 2^13 voxels (< 0.02 voxel) and the error of the reciprocal
+// estimates (relative 2^-22 of at most 64 steps), so each skipped step's tap
+// would have read 1.0 too. The skipped additions are performed one by
+// one (a closed form per binade, vs_fprep.hpp, costs more VALU than the
+// handful of additions a box skips on average).
 constexpr int kSkipMax = 64;
 constexpr float kSkipMargin = 0.25f;
 
@@ -396,8 +412,9 @@ __device__ __forceinline__ int steps_in_box(f3 v, f3 d, int3 lo, int size, const
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float hi = (float)min(ll[a] + size - 1, vdim[a] - 1);
-        if (dd[a] > 0.0f) k = fminf(k, (hi + 0.5f - kSkipMargin - vv[a]) / dd[a]);
-        else if (dd[a] < 0.0f) k = fminf(k, ((float)ll[a] - 0.5f + kSkipMargin - vv[a]) / dd[a]);
+        const float bound = dd[a] > 0.0f ? hi + 0.5f - kSkipMargin : (float)ll[a] - 0.5f + kSkipMargin;
+        const float t = (bound - vv[a]) * __builtin_amdgcn_rcpf(dd[a]);
+        k = dd[a] != 0.0f ? fminf(k, t) : k;
     }
     return k >= 1.0f ? (int)k : 0;
 }
@@ -409,9 +426,11 @@ __device__ __forceinline__ int steps_outside(f3 v, f3 d, const int *vdim) {
     const float vv[3] = {v.x, v.y, v.z}, dd[3] = {d.x, d.y, d.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (vv[a] <= -0.5f && dd[a] > 0.0f) k = fmaxf(k, (-0.5f - kSkipMargin - vv[a]) / dd[a]);
-        else if (vv[a] >= (float)vdim[a] - 0.5f && dd[a] < 0.0f)
-            k = fmaxf(k, ((float)vdim[a] - 0.5f + kSkipMargin - vv[a]) / dd[a]);
+        const bool below = vv[a] <= -0.5f && dd[a] > 0.0f;
+        const bool above = vv[a] >= (float)vdim[a] - 0.5f && dd[a] < 0.0f;
+        const float bound = below ? -0.5f - kSkipMargin : (float)vdim[a] - 0.5f + kSkipMargin;
+        const float t = (bound - vv[a]) * __builtin_amdgcn_rcpf(dd[a]);
+        k = (below || above) ? fmaxf(k, t) : k;
     }
     k = fminf(k, (float)kSkipMax);
     return k >= 1.0f ? (int)k : 0;
@@ -429,7 +448,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
     world_to_voxel(V.P, ps, vox);
     float tsdf;
     if (STATS) ++n_near;
-    const bool valid = V.nearest(vox, rd, tsdf) > 0;
+    const bool valid = V.nearest_valid(vox, tsdf);
     bool coarse;
     if (!valid) coarse = true;
     else if (tsdf >= 0.0f) coarse = false;
@@ -477,7 +496,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         int size;
         if (V.nearest(vox, rd, tsdf, lo, size) < 0 && !STATS) return false;
         if (!STATS && size != 0) {
-            // no data here and in the region around: skip the steps that stay
+            // 1.0 here and in the region around: skip the steps that stay
             // inside it (tsdf is 1.0, so the next step is the same S each time)
             const float S = coarse ? C.step_scale : fmaxf(C.step_scale * tsdf, 1.0f);
             const f3 d = scale(S, rd);
@@ -646,23 +665,29 @@ __device__ __forceinline__ WorkItem level_item(const LevelWork &lw, int &level) 
 }
 
 template <class TV, class TW, bool STATS>
-__global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__restrict__ SP, LevelWork lw,
-                                                          unsigned long long *stats) {
-    __shared__ uint32_t s_socc[kSoccLdsWords];
+// 8 waves per SIMD (at most 64 VGPRs): the march is VALU-bound with dependent
+// taps, and the extra waves hide the taps' latency better than the few
+// registers the compiler would add beyond 64.
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_raycast(
+    const SessParams *__restrict__ SP, LevelWork lw, unsigned long long *stats) {
+    __shared__ uint32_t s_sband[kSbandLdsWords];
     int level;
     const WorkItem wi = level_item(lw, level);
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
     const LevelDesc &L = P.lev[level];
     if (!st->fuse_ok || level >= P.levels || tile_out(L, wi.tile)) return;
-    if (P.socc_words <= kSoccLdsWords) {
-        for (int i = threadIdx.x; i < P.socc_words; i += kTileThreads) s_socc[i] = ldg(P.socc, i);
+    if (P.sband_words <= kSbandLdsWords) {
+        for (int i = threadIdx.x; i < P.sband_words; i += kTileThreads) s_sband[i] = ldg(P.sband, i);
     } else {
-        for (int i = threadIdx.x; i < kSoccLdsWords; i += kTileThreads) s_socc[i] = 0xFFFFFFFFu;
+        for (int i = threadIdx.x; i < kSbandLdsWords; i += kTileThreads) s_sband[i] = 0xFFFFFFFFu;
     }
     __syncthreads();
-    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
-    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    // each wave marches an 8x8 block of the 32x8 tile (neighbouring rays
+    // take similar paths, so a wave's lanes diverge less than on 32x2 rows)
+    const int lane = threadIdx.x & 63;
+    const int x = (wi.tile % L.tiles_x) * kTileW + (threadIdx.x >> 6) * 8 + (lane & 7);
+    const int y = (wi.tile / L.tiles_x) * kTileH + (lane >> 3);
     if (x >= L.W || y >= L.H) return;
     RayConst C;
 #pragma unroll
@@ -675,7 +700,7 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast(const SessParams *__re
     C.step_scale = P.step_scale;
     C.total_max = P.total_max;
     C.max_iter = P.max_iter;
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint32_t *)s_socc};
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint32_t *)s_sband};
     unsigned n_near = 0, n_tri = 0;
     f3 p;
     const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri);
@@ -752,13 +777,14 @@ __global__ void k_volume_init(const SessParams *__restrict__ SP, size_t nvox, ui
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (size_t)gridDim.x * blockDim.x) {
         stg(P.bvis, w, 0u);
         stg(P.bocc, w, 0u);
-        if (w < (size_t)P.socc_words) stg(P.socc, w, 0u);
+        stg(P.bband, w, 0u);
+        if (w < (size_t)P.sband_words) stg(P.sband, w, 0u);
     }
 }
 
-__global__ void k_socc_clear(const SessParams *__restrict__ SP) {
+__global__ void k_sband_clear(const SessParams *__restrict__ SP) {
     const SessParams &P = SP[0];
-    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < P.socc_words; w += gridDim.x * blockDim.x) stg(P.socc, (size_t)w, 0u);
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < P.sband_words; w += gridDim.x * blockDim.x) stg(P.sband, (size_t)w, 0u);
 }
 
 // brick-major -> linear x-fastest, converted to float
@@ -844,6 +870,7 @@ __global__ __launch_bounds__(kTileThreads) void k_carve(const SessParams *__rest
         if (t == 0) {
             atomicAnd(P.bvis + (b >> 5), ~(1u << (b & 31)));
             atomicAnd(P.bocc + (b >> 5), ~(1u << (b & 31)));
+            atomicAnd(P.bband + (b >> 5), ~(1u << (b & 31)));
             atomicAdd(carved, 1u);
         }
     }
@@ -877,21 +904,34 @@ __global__ void k_volume_from_linear(const SessParams *__restrict__ SP, size_t n
         st_w((TW *)P.weight, j, Wt[i]);
     }
 }
-// bocc from the weights: one workgroup per brick
-template <class TW>
+// bocc and the band bits from the volume (after an upload or a restore): one
+// workgroup per brick; sband is cleared first
+template <class TV, class TW>
 __global__ __launch_bounds__(kTileThreads) void k_occupancy(const SessParams *__restrict__ SP, uint32_t nbricks) {
     const SessParams &P = SP[0];
     for (uint32_t b = blockIdx.x; b < nbricks; b += gridDim.x) {
         const TW *Wt = (const TW *)P.weight + (size_t)b * kBrickVox;
-        const int any = __syncthreads_or((ld_w(Wt, threadIdx.x) > 0.0f) || (ld_w(Wt, threadIdx.x + 256) > 0.0f));
+        const TV *F = (const TV *)P.tsdf + (size_t)b * kBrickVox;
+        bool any = false, band = false;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = threadIdx.x + kTileThreads * h;
+            const bool w = ld_w(Wt, i) > 0.0f;
+            any |= w;
+            band |= w && ld_val(F, i) != 1.0f;
+        }
+        const int cany = __syncthreads_or(any), cband = __syncthreads_or(band);
         if (threadIdx.x == 0) {
-            if (any) {
-                atomicOr(P.bocc + (b >> 5), 1u << (b & 31));
+            const uint32_t bb = 1u << (b & 31);
+            if (cany) atomicOr(P.bocc + (b >> 5), bb);
+            else atomicAnd(P.bocc + (b >> 5), ~bb);
+            if (cband) {
+                atomicOr(P.bband + (b >> 5), bb);
                 const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
                 const int sb = super_of_brick(P, bx, by, bz);
-                atomicOr(P.socc + (sb >> 5), 1u << (sb & 31));
+                atomicOr(P.sband + (sb >> 5), 1u << (sb & 31));
             } else {
-                atomicAnd(P.bocc + (b >> 5), ~(1u << (b & 31)));
+                atomicAnd(P.bband + (b >> 5), ~bb);
             }
         }
     }
@@ -1008,12 +1048,15 @@ void launch_volume_from_linear(const SessParams *P, size_t nvox, int tt, int wt,
                                hipStream_t s) {
     VS_DISPATCH(tt, wt, from_lin_t, P, nvox, F, Wt, s);
 }
-void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int wt, hipStream_t s) {
+template <class TV, class TW>
+static void occupancy_t(const SessParams *P, uint32_t nbricks, int g, hipStream_t s) {
+    hipLaunchKernelGGL((k_occupancy<TV, TW>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
+}
+void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tt, int wt, hipStream_t s) {
     int g = (int)(nbricks > 16384 ? 16384 : nbricks);
     if (g <= 0) return;
-    hipLaunchKernelGGL(k_socc_clear, dim3(16), dim3(256), 0, s, P);
-    if (wt == 0) hipLaunchKernelGGL((k_occupancy<float>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
-    else hipLaunchKernelGGL((k_occupancy<uint8_t>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
+    hipLaunchKernelGGL(k_sband_clear, dim3(16), dim3(256), 0, s, P);
+    VS_DISPATCH(tt, wt, occupancy_t, P, nbricks, g, s);
 }
 
 template <class TV, class TW>
