@@ -188,6 +188,9 @@ def serve(ctx, streams, bufs, frames, W, H, seconds, with_plain):
     return res
 
 
+TUNE_STEPS = 13  # the tuner's 12 candidate steps + the step that settles the shapes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +208,8 @@ def main():
                     help="end with one device-to-device copy of this size (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-plain", type=int, default=1,
                     help="also run the same request trace through the plain serial pipeline")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="fixed launch shapes instead of the online autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build shards sessions across GPUs only by running one process per "
@@ -212,15 +217,30 @@ def main():
         sys.exit(2)
     W, H = 640, 480
     K, Wm = args.steps, args.warmup
-    frames = Wm + K + 2
+    T = 0 if args.no_tune else TUNE_STEPS
+    frames = T + Wm + K + 2
     log(f"bench: {args.sessions} sessions, {W}x{H}, {args.volume}^3 FP16+u8, warmup {Wm}, steps {K}")
     ctx = vscan.Context(0)
     sessions, streams, bufs = make_sessions(ctx, args.sessions, frames, W, H, args.volume)
     stream = hiprt.Stream()
 
     def step(k):
-        ctx.fuse_batch(sessions, frame_ptrs(bufs, k, W, H), is_host=False, stream=stream.h)
+        ctx.fuse_batch(sessions, frame_ptrs(bufs, T + k, W, H), is_host=False, stream=stream.h)
 
+    tune_s = 0.0
+    if T:
+        # launch-shape autotuning: the first steps of the batch's size class
+        # time every candidate shape (setup, before the warmup)
+        log(f"tuning launch shapes ({T} steps)")
+        ctx.set_tuning(vscan.VS_TUNE_ONLINE)
+        t0 = time.perf_counter()
+        for k in range(T):
+            ctx.fuse_batch(sessions, frame_ptrs(bufs, k, W, H), is_host=False, stream=stream.h)
+        stream.sync()
+        tune_s = time.perf_counter() - t0
+        # the settled shapes stay; the later runs (plain pipeline, serving)
+        # tune nothing inside their timed regions
+        ctx.set_tuning(vscan.VS_TUNE_CACHED)
     log("warmup")
     for k in range(Wm):
         step(k)
@@ -257,6 +277,10 @@ def main():
                                      "arrive when it starts)"},
         "gpu_ms_per_step": round(gpu_ms / K, 3),
     }
+    if T:
+        out["launch_shapes"] = {"tuning_steps": T, "tuning_s": round(tune_s, 3), "classes": ctx.tuning(),
+                                "definition": "shapes the online autotuner settled on per size class; ms = mean "
+                                              "stage time of each candidate over its tuning steps"}
 
     if not args.no_roofline:
         # measurement pass: eager launches with events between kernels

@@ -226,6 +226,32 @@ vs_status vs_get_stage_ms(vs_session *s, float ms[5]);
  * levels, one launch), 0] in ms (waits for the step). */
 vs_status vs_ctx_set_timing(vs_ctx *ctx, int enable);
 vs_status vs_ctx_stage_ms(vs_ctx *ctx, float ms[8]);
+
+/* Launch-shape autotuning (no counterpart in the reference, whose launches
+ * have fixed shapes). Three launches of a step have interchangeable shapes:
+ * the ray block of a wave in the raycast march and in its shading (8x8, 16x4
+ * or 32x2 rays) and the workgroup count of the persistent integrate (1024,
+ * 2048, 4096 or 8192); every shape gives bit-identical results. With
+ * VS_TUNE_ONLINE, a batch (vs_fuse_batch, a server's slot table, a session's
+ * own vs_fuse_frame) whose size class (largest frame, largest volume edge
+ * and session count rounded up to powers of two, storage types) has no
+ * choice yet runs its first 12 steps eagerly with the candidates in turn,
+ * timed per stage (those steps ignore vs_ctx_set_timing), then keeps the
+ * fastest shape per stage for that class. VS_TUNE_CACHED applies the shapes
+ * settled so far (in this context or read from the cache file) and tunes
+ * nothing. VS_TUNE_OFF (the default) tunes nothing and applies nothing: batches
+ * not settled earlier use fixed shapes (march 8x8, shade 32x2, integrate 8192
+ * workgroups, 2048 for a session's own steps). cache_path (nullable) names a
+ * text file the choices are read from now and written to after each tuning. */
+typedef enum { VS_TUNE_OFF = 0, VS_TUNE_ONLINE = 1, VS_TUNE_CACHED = 2 } vs_tune_mode;
+vs_status vs_ctx_set_tuning(vs_ctx *ctx, int mode, const char *cache_path);
+/* The context's settled shapes, one size class per line:
+ * "<class>|<march block>|<shade block>|<integrate workgroups>|<means>", where
+ * <means> is "m8x8,m16x4,m32x2|s32x2,s8x8|i1024,i2048,i4096,i8192" (mean stage
+ * ms per candidate, -1 for one not timed) or "cached" (from the file). Writes
+ * at most len bytes including the terminating zero; *needed (nullable) = the
+ * full length + 1. */
+vs_status vs_ctx_tuning(vs_ctx *ctx, char *buf, size_t len, size_t *needed);
 /* Counts from the last integrate: visible bricks and updated voxels. */
 vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long long *updated_voxels);
 /* Tap counts of one instrumented raycast at `level` with the current pose and

@@ -161,23 +161,6 @@ vs_status check_desc(const vs_session_desc *d) {
 // ---------------------------------------------------------------------------
 // context, batches
 // ---------------------------------------------------------------------------
-struct Batch {
-    int n = 0, tt = 0, wt = 0, levels = 0;
-    int max_iters[kMaxLevels] = {0, 0, 0};
-    SessParams *dP = nullptr;  // device array
-    WorkItem *dWork[kMaxLevels] = {nullptr, nullptr, nullptr};   // (slot, tile)
-    int nWork[kMaxLevels] = {0, 0, 0};
-    WorkItem *dGroup[kMaxLevels] = {nullptr, nullptr, nullptr};  // (slot, ICP tile group)
-    int nGroup[kMaxLevels] = {0, 0, 0};
-    BrickList bl{};
-    int integ_grid = 1;
-    void *owned = nullptr;  // allocation owned by a context batch (null for a session's own)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<vs_session *> members;
-    const int *active = nullptr;  // slot table: device flags of the slots with a frame this step
-};
-
 // Events around the stages of one frame step:
 //   [0,1) track setup + bilateral   [1,2) pyramid    [2,3) ICP + finalize
 //   [3,4) brick clear + mark        [4,5) integrate  [5,6) raycast march, all levels
@@ -200,6 +183,61 @@ struct Timer {
     }
 };
 
+// ---------------------------------------------------------------------------
+// launch shapes and their autotuner
+// ---------------------------------------------------------------------------
+// The shape of three launches of the step: the ray block of a wave in the
+// raycast march and in its shading (RayBlock), and the workgroup count of the
+// persistent integrate. Every shape gives the same results; which is fastest
+// depends on the workload, so a context can pick them per size class of
+// batch (frame size, volume size, storage types, session count).
+struct Shape {
+    int march = kRay8x8, shade = kRay32x2;
+    int integ = -1;  // index into kIntegGrids; -1: the batch's default grid
+};
+constexpr int kMarchOpts[] = {kRay8x8, kRay16x4, kRay32x2};
+constexpr int kShadeOpts[] = {kRay32x2, kRay8x8};
+constexpr int kIntegGrids[] = {1024, 2048, 4096, 8192};
+constexpr int kNumMarch = 3, kNumShade = 2, kNumInteg = 4;
+const char *const kRayBlockName[] = {"8x8", "16x4", "32x2"};
+// Online tuning: the first kTuneSteps steps of a batch whose size class has
+// no choice yet run eagerly and timed, step i with march candidate i % 3,
+// shade candidate i % 2 and integrate grid i % 4 (each candidate of each
+// stage is timed 2 to 6 times, on consecutive frames); then each stage keeps
+// the candidate with the lowest mean time and the step is captured with it.
+constexpr int kTuneSteps = 12;
+struct TuneState {
+    int step = 0;           // tuning steps enqueued
+    double sum[3][4] = {};  // [march, shade, integrate][candidate] ms
+    int cnt[3][4] = {};
+    int ci[3] = {0, 0, 0};  // candidates of the step in flight
+    bool pending = false;   // that step's events are not read yet
+    Timer timer;
+    ~TuneState() { timer.destroy(); }
+};
+
+struct Batch {
+    int n = 0, tt = 0, wt = 0, levels = 0;
+    int max_iters[kMaxLevels] = {0, 0, 0};
+    SessParams *dP = nullptr;  // device array
+    WorkItem *dWork[kMaxLevels] = {nullptr, nullptr, nullptr};   // (slot, tile)
+    int nWork[kMaxLevels] = {0, 0, 0};
+    WorkItem *dGroup[kMaxLevels] = {nullptr, nullptr, nullptr};  // (slot, ICP tile group)
+    int nGroup[kMaxLevels] = {0, 0, 0};
+    BrickList bl{};
+    uint32_t bricks = 0;    // capacity of the brick list
+    int integ_grid = 1;     // default integrate grid
+    Shape shape;
+    std::string size_class;
+    bool tuned = false;     // shape settled (from the tuner or its cache)
+    std::unique_ptr<TuneState> tune;
+    void *owned = nullptr;  // allocation owned by a context batch (null for a session's own)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<vs_session *> members;
+    const int *active = nullptr;  // slot table: device flags of the slots with a frame this step
+};
+
 struct vs_ctx {
     int device = 0;
     std::map<std::vector<vs_session *>, std::unique_ptr<Batch>> batches;
@@ -208,6 +246,10 @@ struct vs_ctx {
     bool use_graph = true;
     bool timing = false;
     Timer timer;
+    int tune_mode = VS_TUNE_OFF;
+    std::string tune_path;                 // cache file ("" = none)
+    std::map<std::string, Shape> tuned;    // size class -> shape
+    std::map<std::string, std::string> tune_ms;  // size class -> candidate means
 };
 
 struct vs_session {
@@ -256,7 +298,12 @@ struct ListSet {
     std::vector<WorkItem> tile[kMaxLevels], group[kMaxLevels];
     uint32_t bricks = 0;
     int levels = 0, max_iters[kMaxLevels] = {0, 0, 0};
+    int count = 0, max_w = 0, max_h = 0, max_dim = 0;
     void add(int slot, const vs_session *s) {
+        ++count;
+        max_w = std::max(max_w, s->d.width);
+        max_h = std::max(max_h, s->d.height);
+        for (int k = 0; k < 3; ++k) max_dim = std::max(max_dim, s->d.vol_dim[k]);
         levels = std::max(levels, s->d.levels);
         for (int l = 0; l < kMaxLevels; ++l) {
             max_iters[l] = std::max(max_iters[l], s->hp.iters[l]);
@@ -297,10 +344,32 @@ struct ListSet {
         HIP_TRY(hipMemset(B.bl.count, 0, 16));
         B.levels = levels;
         for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = max_iters[l];
+        B.bricks = bricks;
         B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(bricks, (uint32_t)integ_max));
+        // size class: largest frame, largest volume edge and session count
+        // rounded up to powers of two, storage types
+        auto pow2 = [](int v) {
+            int p = 1;
+            while (p < v) p <<= 1;
+            return p;
+        };
+        char buf[96];
+        snprintf(buf, sizeof(buf), "%dx%d v%d %s+%s n%d", max_w, max_h, pow2(max_dim), B.tt ? "f16" : "f32",
+                 B.wt ? "u8" : "f32", pow2(count));
+        if (B.size_class != buf) {
+            B.size_class = buf;
+            B.tuned = false;
+            B.tune.reset();
+        }
         return VS_OK;
     }
 };
+
+// Workgroups of the integrate launch for the batch's shape.
+int integ_grid(const Batch &B) {
+    if (B.shape.integ < 0) return B.integ_grid;
+    return (int)std::max<uint32_t>(1, std::min<uint32_t>(B.bricks, (uint32_t)kIntegGrids[B.shape.integ]));
+}
 
 void free_batch_graph(Batch &b) {
     if (b.gexec) hipGraphExecDestroy(b.gexec);
@@ -441,18 +510,137 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     launch_vis_clear(B.dP, B.n, B.bl, s);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, s);
     mark(4);
-    launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, s);
+    launch_integrate(B.dP, B.bl, B.tt, B.wt, integ_grid(B), s);
     mark(5);
     const LevelWork lw = level_work(B.dWork, B.nWork, B.levels);
-    launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, s);
+    launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, s);
     mark(6);
-    launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, s);
+    launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, B.shape.shade, s);
     mark(7);
     mark(8);
     if (timed) timed->used = true;
 }
 
+std::string shape_text(const Shape &h, const Batch &B) {
+    char buf[96];
+    snprintf(buf, sizeof(buf), "march=%s shade=%s integrate=%d", kRayBlockName[h.march], kRayBlockName[h.shade],
+             h.integ < 0 ? B.integ_grid : kIntegGrids[h.integ]);
+    return buf;
+}
+
+// Writes the context's settled shapes to its cache file, one size class a
+// line: "<class>|<march block>|<shade block>|<integrate grid>".
+void save_tuning(const vs_ctx *ctx) {
+    if (ctx->tune_path.empty()) return;
+    FILE *f = fopen(ctx->tune_path.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "# vscan launch shapes v1: size class|march ray block|shade ray block|integrate workgroups\n");
+    for (const auto &kv : ctx->tuned)
+        fprintf(f, "%s|%s|%s|%d\n", kv.first.c_str(), kRayBlockName[kv.second.march], kRayBlockName[kv.second.shade],
+                kIntegGrids[kv.second.integ]);
+    fclose(f);
+}
+
+int block_index(const char *name) {
+    for (int i = 0; i < 3; ++i)
+        if (!strcmp(name, kRayBlockName[i])) return i;
+    return -1;
+}
+
+void load_tuning(vs_ctx *ctx) {
+    FILE *f = fopen(ctx->tune_path.c_str(), "r");
+    if (!f) return;
+    char line[256];
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#') continue;
+        char *fld[4];
+        int nf = 0;
+        for (char *p = strtok(line, "|\r\n"); p && nf < 4; p = strtok(nullptr, "|\r\n")) fld[nf++] = p;
+        if (nf != 4) continue;
+        Shape h;
+        h.march = block_index(fld[1]);
+        h.shade = block_index(fld[2]);
+        const int g = atoi(fld[3]);
+        for (int i = 0; i < kNumInteg; ++i)
+            if (kIntegGrids[i] == g) h.integ = i;
+        if (h.march < 0 || h.shade < 0 || h.integ < 0) continue;
+        ctx->tuned[fld[0]] = h;
+    }
+    fclose(f);
+}
+
+// One step of online tuning: reads the stage times of the previous tuning
+// step, then enqueues the next candidate step (eager, timed), or after
+// kTuneSteps settles the batch's shape and leaves the step to the caller.
+vs_status tune_step(vs_ctx *ctx, Batch &B, hipStream_t s) {
+    if (!B.tune) B.tune.reset(new TuneState());
+    TuneState &T = *B.tune;
+    if (T.pending) {
+        HIP_TRY(hipEventSynchronize(T.timer.ev[kMarks - 1]));
+        float ms[3];
+        HIP_TRY(hipEventElapsedTime(&ms[0], T.timer.ev[5], T.timer.ev[6]));  // march
+        HIP_TRY(hipEventElapsedTime(&ms[1], T.timer.ev[6], T.timer.ev[7]));  // shade
+        HIP_TRY(hipEventElapsedTime(&ms[2], T.timer.ev[4], T.timer.ev[5]));  // integrate
+        for (int k = 0; k < 3; ++k) {
+            T.sum[k][T.ci[k]] += ms[k];
+            T.cnt[k][T.ci[k]] += 1;
+        }
+        T.pending = false;
+    }
+    if (T.step < kTuneSteps) {
+        T.ci[0] = T.step % kNumMarch;
+        T.ci[1] = T.step % kNumShade;
+        T.ci[2] = T.step % kNumInteg;
+        B.shape.march = kMarchOpts[T.ci[0]];
+        B.shape.shade = kShadeOpts[T.ci[1]];
+        B.shape.integ = T.ci[2];
+        HIP_TRY(T.timer.init());
+        enqueue_step(B, s, &T.timer);
+        HIP_TRY(hipGetLastError());
+        T.pending = true;
+        ++T.step;
+        return VS_OK;
+    }
+    const int nopt[3] = {kNumMarch, kNumShade, kNumInteg};
+    int best[3] = {0, 0, 0};
+    std::string ms;
+    for (int k = 0; k < 3; ++k) {
+        double bm = 1e30;
+        for (int c = 0; c < nopt[k]; ++c) {
+            const double m = T.cnt[k][c] ? T.sum[k][c] / T.cnt[k][c] : 1e30;
+            if (m < bm) {
+                bm = m;
+                best[k] = c;
+            }
+            char b[32];
+            snprintf(b, sizeof(b), "%s%.4f", c ? "," : (k ? "|" : ""), T.cnt[k][c] ? m : -1.0);
+            ms += b;
+        }
+    }
+    B.shape.march = kMarchOpts[best[0]];
+    B.shape.shade = kShadeOpts[best[1]];
+    B.shape.integ = best[2];
+    B.tuned = true;
+    B.tune.reset();
+    free_batch_graph(B);
+    ctx->tuned[B.size_class] = B.shape;
+    ctx->tune_ms[B.size_class] = ms;
+    save_tuning(ctx);
+    return VS_OK;
+}
+
 vs_status run_step(vs_ctx *ctx, Batch &B, hipStream_t s, Timer *timed) {
+    if (ctx->tune_mode != VS_TUNE_OFF && !B.tuned) {
+        auto it = ctx->tuned.find(B.size_class);
+        if (it != ctx->tuned.end()) {
+            B.shape = it->second;
+            B.tuned = true;
+            free_batch_graph(B);
+        } else if (ctx->tune_mode == VS_TUNE_ONLINE) {
+            vs_status r = tune_step(ctx, B, s);
+            if (r != VS_OK || !B.tuned) return r;  // a tuning step was enqueued
+        }
+    }
     if (timed || !ctx->use_graph) {
         if (timed) HIP_TRY(timed->init());
         enqueue_step(B, s, timed);
@@ -832,7 +1020,7 @@ vs_status vs_integrate(vs_session *s, vs_stream stream) {
     launch_fuse_ok(B.dP, 1, h);
     launch_vis_clear(B.dP, 1, B.bl, h);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, h);
-    launch_integrate(B.dP, B.bl, B.tt, B.wt, B.integ_grid, h);
+    launch_integrate(B.dP, B.bl, B.tt, B.wt, integ_grid(B), h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
 }
@@ -1338,6 +1526,38 @@ vs_status vs_get_stage_ms(vs_session *s, float ms[5]) {
     ms[2] = k[2];                 // track
     ms[3] = k[3] + k[4];          // integrate (visibility + update)
     ms[4] = k[5] + k[6] + k[7];   // raycast, all levels
+    return VS_OK;
+}
+
+vs_status vs_ctx_set_tuning(vs_ctx *ctx, int mode, const char *cache_path) {
+    if (!ctx) return fail(VS_EINVAL, "null context");
+    if (mode != VS_TUNE_OFF && mode != VS_TUNE_ONLINE && mode != VS_TUNE_CACHED)
+        return fail(VS_EINVAL, "tuning mode %d", mode);
+    ctx->tune_mode = mode;
+    ctx->tune_path = cache_path ? cache_path : "";
+    if (!ctx->tune_path.empty()) load_tuning(ctx);
+    return VS_OK;
+}
+
+vs_status vs_ctx_tuning(vs_ctx *ctx, char *buf, size_t len, size_t *needed) {
+    if (!ctx) return fail(VS_EINVAL, "null context");
+    std::string out;
+    for (const auto &kv : ctx->tuned) {
+        const Shape &h = kv.second;
+        char line[160];
+        snprintf(line, sizeof(line), "%s|%s|%s|%d|", kv.first.c_str(), kRayBlockName[h.march],
+                 kRayBlockName[h.shade], kIntegGrids[h.integ]);
+        out += line;
+        auto m = ctx->tune_ms.find(kv.first);
+        out += m != ctx->tune_ms.end() ? m->second : std::string("cached");
+        out += "\n";
+    }
+    if (needed) *needed = out.size() + 1;
+    if (buf && len) {
+        const size_t n = std::min(len - 1, out.size());
+        memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    }
     return VS_OK;
 }
 

@@ -57,10 +57,12 @@ struct LevelWork {
     }
 };
 LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels);
+// Ray block of one wave in the raycast kernels (same results for each).
+enum RayBlock { kRay8x8 = 0, kRay16x4 = 1, kRay32x2 = 2 };
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
-                          unsigned long long *stats, hipStream_t s);
+                          unsigned long long *stats, int ray_block, hipStream_t s);
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
-                          unsigned long long *stats, hipStream_t s);
+                          unsigned long long *stats, int ray_block, hipStream_t s);
 // volume maintenance (single session, P points at its parameters)
 void launch_volume_init(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, uint32_t nwords,
                         hipStream_t s);

@@ -664,10 +664,22 @@ __device__ __forceinline__ WorkItem level_item(const LevelWork &lw, int &level) 
     return lw.w[s][b];
 }
 
-template <class TV, class TW, bool STATS>
+// Pixel of thread t inside its 32x8 tile when each wave takes a block of
+// BW x (64 / BW) rays (RayBlock: 8x8, 16x4 or 32x2). Neighbouring rays take
+// similar paths, so squarer blocks diverge less; which block is fastest for a
+// stage and workload is the autotuner's choice (the results are the same).
+template <int RB>
+__device__ __forceinline__ void ray_pixel(int t, int &px, int &py) {
+    constexpr int BW = RB == kRay8x8 ? 8 : RB == kRay16x4 ? 16 : 32, BH = 64 / BW, PER_ROW = kTileW / BW;
+    const int w = t >> 6, lane = t & 63;
+    px = (w % PER_ROW) * BW + lane % BW;
+    py = (w / PER_ROW) * BH + lane / BW;
+}
+
 // 8 waves per SIMD (at most 64 VGPRs): the march is VALU-bound with dependent
 // taps, and the extra waves hide the taps' latency better than the few
 // registers the compiler would add beyond 64.
+template <class TV, class TW, bool STATS, int RB>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_raycast(
     const SessParams *__restrict__ SP, LevelWork lw, unsigned long long *stats) {
     __shared__ uint32_t s_sband[kSbandLdsWords];
@@ -683,11 +695,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
         for (int i = threadIdx.x; i < kSbandLdsWords; i += kTileThreads) s_sband[i] = 0xFFFFFFFFu;
     }
     __syncthreads();
-    // each wave marches an 8x8 block of the 32x8 tile (neighbouring rays
-    // take similar paths, so a wave's lanes diverge less than on 32x2 rows)
-    const int lane = threadIdx.x & 63;
-    const int x = (wi.tile % L.tiles_x) * kTileW + (threadIdx.x >> 6) * 8 + (lane & 7);
-    const int y = (wi.tile / L.tiles_x) * kTileH + (lane >> 3);
+    int px, py;
+    ray_pixel<RB>(threadIdx.x, px, py);
+    const int x = (wi.tile % L.tiles_x) * kTileW + px;
+    const int y = (wi.tile / L.tiles_x) * kTileH + py;
     if (x >= L.W || y >= L.H) return;
     RayConst C;
 #pragma unroll
@@ -719,7 +730,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     }
 }
 
-template <class TV, class TW, bool STATS>
+template <class TV, class TW, bool STATS, int RB>
 __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams *__restrict__ SP, LevelWork lw,
                                                                 unsigned long long *stats) {
     int level;
@@ -728,8 +739,10 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
     const SessState *st = P.st;
     const LevelDesc &L = P.lev[level];
     if (!st->fuse_ok || level >= P.levels || tile_out(L, wi.tile)) return;
-    const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
-    const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+    int px, py;
+    ray_pixel<RB>(threadIdx.x, px, py);
+    const int x = (wi.tile % L.tiles_x) * kTileW + px;
+    const int y = (wi.tile / L.tiles_x) * kTileH + py;
     if (x >= L.W || y >= L.H) return;
     const size_t t = (size_t)y * L.W + x;
     const float4 h = ldg4(P.mvtx[level], t);
@@ -964,22 +977,31 @@ void launch_integrate(const SessParams *P, BrickList bl, int tt, int wt, int gri
 }
 
 template <class TV, class TW>
-static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, hipStream_t s) {
+static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, hipStream_t s) {
     const int n = lw.blocks();
     if (n <= 0) return;
     if (stats)
-        hipLaunchKernelGGL((k_raycast<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        hipLaunchKernelGGL((k_raycast<TV, TW, true, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+    else if (rb == kRay16x4)
+        hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay16x4>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+    else if (rb == kRay32x2)
+        hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay32x2>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
     else
-        hipLaunchKernelGGL((k_raycast<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
 }
 template <class TV, class TW>
-static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, hipStream_t s) {
+static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, hipStream_t s) {
     const int n = lw.blocks();
     if (n <= 0) return;
     if (stats)
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true, kRay32x2>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
+                           stats);
+    else if (rb == kRay8x8)
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
+                           stats);
     else
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, kRay32x2>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
+                           stats);
 }
 LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels) {
     LevelWork lw{};
@@ -993,12 +1015,12 @@ LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLev
     return lw;
 }
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
-                          hipStream_t s) {
-    VS_DISPATCH(tt, wt, march_t, P, lw, stats, s);
+                          int rb, hipStream_t s) {
+    VS_DISPATCH(tt, wt, march_t, P, lw, stats, rb, s);
 }
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
-                          hipStream_t s) {
-    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, s);
+                          int rb, hipStream_t s) {
+    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, rb, s);
 }
 static LevelWork one_level(const WorkItem *work, int n, int level) {
     LevelWork lw{};
@@ -1012,14 +1034,14 @@ static LevelWork one_level(const WorkItem *work, int n, int level) {
 }
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
-    launch_raycast_march(P, lw, tt, wt, nullptr, s);
-    launch_raycast_shade(P, lw, tt, wt, nullptr, s);
+    launch_raycast_march(P, lw, tt, wt, nullptr, kRay8x8, s);
+    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay32x2, s);
 }
 void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
                           unsigned long long *stats, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
-    launch_raycast_march(P, lw, tt, wt, stats, s);
-    launch_raycast_shade(P, lw, tt, wt, stats, s);
+    launch_raycast_march(P, lw, tt, wt, stats, kRay8x8, s);
+    launch_raycast_shade(P, lw, tt, wt, stats, kRay32x2, s);
 }
 
 static int grid_for(size_t n) {

@@ -19,6 +19,7 @@ VS_OK, VS_EINVAL, VS_ENOMEM, VS_EHIP, VS_TRACK_LOST, VS_MOTION_REJECTED, VS_BUDG
 VS_TSDF_F32, VS_TSDF_F16 = 0, 1
 VS_WEIGHT_F32, VS_WEIGHT_U8 = 0, 1
 VS_INTEGRATE_DEPTH, VS_INTEGRATE_COLOR = 0, 1
+VS_TUNE_OFF, VS_TUNE_ONLINE, VS_TUNE_CACHED = 0, 1, 2
 
 
 class SessionDesc(C.Structure):
@@ -100,8 +101,10 @@ def lib():
                      "vs_server_step", "vs_server_usage", "vs_set_color", "vs_get_texture",
                      "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
                      "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot", "vs_depth_reproject",
-                     "vs_depth_crop_to_volume", "vs_volume_carve"]:
+                     "vs_depth_crop_to_volume", "vs_volume_carve", "vs_ctx_set_tuning", "vs_ctx_tuning"]:
             getattr(L, name).restype = C.c_int
+        L.vs_ctx_set_tuning.argtypes = [P, C.c_int, C.c_char_p]
+        L.vs_ctx_tuning.argtypes = [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.vs_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.vs_session_create.argtypes = [P, C.POINTER(SessionDesc), P, C.POINTER(P)]
         L.vs_session_reset.argtypes = [P, P, P]
@@ -298,6 +301,30 @@ class Context:
 
     def set_timing(self, on):
         check(lib().vs_ctx_set_timing(self.h, 1 if on else 0), "vs_ctx_set_timing")
+
+    def set_tuning(self, mode=VS_TUNE_ONLINE, cache_path=None):
+        """Launch-shape autotuning (vs_ctx_set_tuning)."""
+        path = cache_path.encode() if cache_path else None
+        check(lib().vs_ctx_set_tuning(self.h, int(mode), path), "vs_ctx_set_tuning")
+
+    def tuning(self):
+        """Settled launch shapes per size class (vs_ctx_tuning): {class: {"march", "shade",
+        "integrate", "ms"}}, ms = {"march": {block: ms}, "shade": {...}, "integrate": {grid: ms}}
+        for classes tuned in this context, None for ones read from a cache file."""
+        need = C.c_size_t()
+        check(lib().vs_ctx_tuning(self.h, None, 0, C.byref(need)), "vs_ctx_tuning")
+        buf = C.create_string_buffer(need.value)
+        check(lib().vs_ctx_tuning(self.h, buf, need.value, None), "vs_ctx_tuning")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            f = line.split("|")
+            ms = None
+            if f[4] != "cached":
+                m, sh, ig = ([float(x) for x in g.split(",")] for g in f[4:7])
+                ms = {"march": dict(zip(["8x8", "16x4", "32x2"], m)), "shade": dict(zip(["32x2", "8x8"], sh)),
+                      "integrate": dict(zip([1024, 2048, 4096, 8192], ig))}
+            out[f[0]] = {"march": f[1], "shade": f[2], "integrate": int(f[3]), "ms": ms}
+        return out
 
     def stage_ms(self):
         ms = np.zeros(8, np.float32)
