@@ -84,9 +84,12 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
                 const int old = atomicCAS(&set[h], -1, (int)b);
                 if (old == -1) {
                     // first sample of this brick in the workgroup: set its bit
+                    // bits are only set during this launch (cleared by the
+                    // launch before): a plain cached read that shows the bit
+                    // proves it set; a stale 0 costs one atomic
                     const uint32_t bit = 1u << (b & 31u);
                     uint32_t *word = P.bvis + (b >> 5);
-                    if (!(__atomic_load_n(word, __ATOMIC_RELAXED) & bit) && !(atomicOr(word, bit) & bit))
+                    if (!(ldg(P.bvis, b >> 5) & bit) && !(atomicOr(word, bit) & bit))
                         fresh[atomicAdd(&nfresh, 1u)] = (int)b;
                     break;
                 }
@@ -192,6 +195,11 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
     const unsigned n = min(*bl.count, bl.capacity);
     const int t = threadIdx.x;
     const int xl = t & 7, yl = (t >> 3) & 7, zl = t >> 6;
+    // per-wave counts of a brick, double-buffered by brick parity so that one
+    // barrier per brick suffices: a wave can be one brick ahead of thread 0,
+    // never two
+    __shared__ unsigned s_upd[2][kTileThreads / 64], s_band[2][kTileThreads / 64];
+    int parity = 0;
     // updated-voxel count of the current run of same-session bricks (thread 0)
     int run_sess = -1;
     unsigned long long run_upd = 0ull;
@@ -243,28 +251,40 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             ++nupd;
         }
         }
-        const int c1 = __syncthreads_count(nupd >= 1), c2 = __syncthreads_count(nupd >= 2);
-        const int cb = __syncthreads_or(band);
-        if (t == 0 && c1 > 0) {
-            // occupancy and band bits only ever get set: skip the device
-            // atomic when a plain read already shows the bit (a stale 0 costs
-            // one atomic)
-            const uint32_t bb = 1u << (b & 31);
-            if (!(__atomic_load_n(P.bocc + (b >> 5), __ATOMIC_RELAXED) & bb)) atomicOr(P.bocc + (b >> 5), bb);
-            if (cb) {
-                if (!(__atomic_load_n(P.bband + (b >> 5), __ATOMIC_RELAXED) & bb)) atomicOr(P.bband + (b >> 5), bb);
+        const unsigned long long w1 = __ballot(nupd >= 1), w2 = __ballot(nupd >= 2), wb = __ballot(band);
+        if ((t & 63) == 0) {
+            s_upd[parity][t >> 6] = (unsigned)(__popcll(w1) + __popcll(w2));
+            s_band[parity][t >> 6] = wb != 0ull;
+        }
+        __syncthreads();
+        if (t == 0) {
+            unsigned upd = 0, cb = 0;
+#pragma unroll
+            for (int w = 0; w < kTileThreads / 64; ++w) {
+                upd += s_upd[parity][w];
+                cb |= s_band[parity][w];
+            }
+            if (upd > 0) {
+                // occupancy and band bits are only ever set during this
+                // launch, and earlier launches' writes (clears included) are
+                // visible at its start: a plain (cached) read that shows a
+                // bit proves it set, and a stale 0 costs one redundant atomic
+                const uint32_t bb = 1u << (b & 31);
                 const int sb = super_of_brick(P, bx, by, bz);
                 const uint32_t sbb = 1u << (sb & 31);
-                if (!(__atomic_load_n(P.sband + (sb >> 5), __ATOMIC_RELAXED) & sbb))
-                    atomicOr(P.sband + (sb >> 5), sbb);
+                const uint32_t wo = ldg(P.bocc, b >> 5), wbb = ldg(P.bband, b >> 5), ws = ldg(P.sband, sb >> 5);
+                if (!(wo & bb)) atomicOr(P.bocc + (b >> 5), bb);
+                if (cb && !(wbb & bb)) atomicOr(P.bband + (b >> 5), bb);
+                if (cb && !(ws & sbb)) atomicOr(P.sband + (sb >> 5), sbb);
+                if (it.x != run_sess) {
+                    if (run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
+                    run_sess = it.x;
+                    run_upd = 0ull;
+                }
+                run_upd += (unsigned long long)upd;
             }
-            if (it.x != run_sess) {
-                if (run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
-                run_sess = it.x;
-                run_upd = 0ull;
-            }
-            run_upd += (unsigned long long)(c1 + c2);
         }
+        parity ^= 1;
     }
     if (t == 0 && run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
 }
