@@ -187,6 +187,24 @@ __device__ __forceinline__ void stg4(float4 *p, size_t i, float x, float y, floa
     ((VS_GLOBAL vs_f4 *)p)[i] = vs_f4{x, y, z, w};
 }
 
+// The top 3 rows of a row-major 4x4 FP32 matrix in HBM (16-byte aligned),
+// loaded as three 16-byte vectors into registers.
+struct M34 {
+    float m[12];
+};
+__device__ __forceinline__ M34 ld_m34(const float *T) {
+    M34 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const vs_f4 v = ((const VS_GLOBAL vs_f4 *)T)[i];
+        r.m[4 * i] = v.x;
+        r.m[4 * i + 1] = v.y;
+        r.m[4 * i + 2] = v.z;
+        r.m[4 * i + 3] = v.w;
+    }
+    return r;
+}
+
 __device__ __forceinline__ float depth_m(uint16_t mm) { return (float)mm * 0.001f; }
 // __float2int_rn (round half to even); non-finite or |v| >= 2^31 is out of range
 __device__ __forceinline__ bool rint_ref(float v, int &out) {

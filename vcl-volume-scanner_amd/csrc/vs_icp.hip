@@ -220,8 +220,8 @@ __device__ __forceinline__ void block_terms(float v[kIcpVals], double *red64) {
 }
 
 __device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *st, int level, int iter) {
-    return !st->do_track || st->icp_failed || st->conv_level == level || iter >= P.iters[level] ||
-           (st->force_single && level != 0);
+    return !ldg(&st->do_track, 0) || ldg(&st->icp_failed, 0) || ldg(&st->conv_level, 0) == level ||
+           iter >= P.iters[level] || (ldg(&st->force_single, 0) && level != 0);
 }
 
 // One Gauss-Newton iteration at `level`, first half: the 29 sums of every
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     const int ntiles = L.tiles_x * L.tiles_y;
     const int t0 = wi.tile * kIcpGroup;
     if (t0 >= ntiles) return;
-    if (threadIdx.x < 16) T21s[threadIdx.x] = st->T_21_f[threadIdx.x];
+    if (threadIdx.x < 16) T21s[threadIdx.x] = ldg(st->T_21_f, threadIdx.x);
     __syncthreads();
     float acc[kIcpVals];
 #pragma unroll
@@ -306,28 +306,37 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
         }
     for (int i = 0; i < 6; ++i) ATb[i] = tot[k++];
     const double cnt = tot[21 + 6];
-    for (int i = 0; i < 36; ++i) st->last_ATA[i] = ATA[i];
-    for (int i = 0; i < 6; ++i) st->last_ATb[i] = ATb[i];
-    st->last_valid = cnt;
-    st->solves += 1;
+    // state through global-qualified accesses (a chain of generic accesses
+    // in one thread waits on both the vector-memory and the scalar counters)
+    double T12[16];
+    for (int i = 0; i < 16; ++i) T12[i] = ldg(st->T_12, i);
+    for (int i = 0; i < 36; ++i) stg(st->last_ATA, i, ATA[i]);
+    for (int i = 0; i < 6; ++i) stg(st->last_ATb, i, ATb[i]);
+    stg(&st->last_valid, 0, cnt);
+    stg(&st->solves, 0, ldg(&st->solves, 0) + 1);
     if (P.icp_min_valid > 0 && cnt < (double)P.icp_min_valid) {
-        st->icp_failed = 7;
+        stg(&st->icp_failed, 0, 7);
         return;
     }
     if (!cholesky_solve(ATA, ATb, x6)) {
-        st->icp_failed = 4;
+        stg(&st->icp_failed, 0, 4);
         return;
     }
     double ninf = 0.0;
     for (int i = 0; i < 6; ++i) ninf = fmax(ninf, fabs(x6[i]));
     if (ninf < 5.0e-7) {
-        st->conv_level = level;
+        stg(&st->conv_level, 0, level);
         return;
     }
-    double E[16];
+    double E[16], T21[16];
     expmap(x6, E);
-    matmul4d(E, st->T_12, st->T_12);
-    publish_T21(st);
+    matmul4d(E, T12, T12);
+    rigid_inverse(T12, T21);
+    for (int i = 0; i < 16; ++i) {
+        stg(st->T_12, i, T12[i]);
+        stg(st->T_21, i, T21[i]);
+        stg(st->T_21_f, i, (float)T21[i]);
+    }
 }
 
 // Per-frame / per-track reset of the tracker state (one thread per session).

@@ -65,7 +65,8 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     const float td = (x < L.W && y < L.H) ? depth_m(ldg(P.raw, (size_t)y * L.W + x)) : 0.0f;
     if (td > 0.0f) {
-        const float *Tcg = st->T_cg_f;
+        const M34 Tc = ld_m34(st->T_cg_f);
+        const float *Tcg = Tc.m;
         const f3 p = transform(Tcg, back_project((float)x, (float)y, L.K, td));
         const f3 pe = transform(Tcg, back_project((float)x, (float)y, L.K, 100.0f));
         const f3 dir = normalised(sub(pe, p));
@@ -212,18 +213,20 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
         TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
         TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
         const LevelDesc &L = P.lev[0];
-        const float *T = st->T_gc_f;
+        const M34 Tg = ld_m34(st->T_gc_f);
+        const float *T = Tg.m;
         const float gx = ((float)(bx * kBrick + xl) + 0.5f) / P.vox_inv + P.origin[0];
         const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
         int nupd = 0;
         bool band = false;  // an updated voxel's new value is not exactly 1.0
         if (P.integ_mode == 1) {
+            const M34 Tc = ld_m34(st->T_cg_f);
             uint32_t *col = P.vcol + (size_t)b * kBrickVox;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int zc = bz * kBrick + zl + 4 * h;
                 const float gz = ((float)zc + 0.5f) / P.vox_inv + P.origin[2];
-                nupd += integrate_voxel_color(P, T, st->T_cg_f, {gx, gy, gz}, F, Wt, col, t + kTileThreads * h, band);
+                nupd += integrate_voxel_color(P, T, Tc.m, {gx, gy, gz}, F, Wt, col, t + kTileThreads * h, band);
             }
         } else {
 #pragma unroll
@@ -734,7 +737,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint32_t *)s_sband};
     unsigned n_near = 0, n_tri = 0;
     f3 p;
-    const bool ok = cast_ray<TV, TW, STATS>(V, C, st->T_cg_f, (float)x, (float)y, p, n_near, n_tri);
+    const M34 Tc = ld_m34(st->T_cg_f);
+    const bool ok = cast_ray<TV, TW, STATS>(V, C, Tc.m, (float)x, (float)y, p, n_near, n_tri);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
         stg4(P.mvtx[level], t, p.x, p.y, p.z, 1.0f);
@@ -771,7 +775,8 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
     const f3 p = {h.x, h.y, h.z};
     f3 n;
     if (surface_normal(V, p, n)) {
-        const float *T = st->T_gc_f;
+        const M34 Tg = ld_m34(st->T_gc_f);
+        const float *T = Tg.m;
         const f3 pc = transform(T, p);
         const f3 nc = {T[0] * n.x + T[1] * n.y + T[2] * n.z, T[4] * n.x + T[5] * n.y + T[6] * n.z,
                        T[8] * n.x + T[9] * n.y + T[10] * n.z};
