@@ -54,7 +54,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, bband, sband, icp_part, diag_part, diag_mask, lists, params, stats, keys;
+    size_t tsdf, weight, bvis, bocc, bband, sband, sdist, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t total;
     size_t nvox;
@@ -112,6 +112,7 @@ Layout make_layout(const vs_session_desc &d) {
     L.bocc = take((size_t)L.nwords * 4);
     L.bband = take((size_t)L.nwords * 4);
     L.sband = take((size_t)L.nswords * 4);
+    L.sdist = take(((size_t)L.nsuper + 7) / 8 * 4);  // 4 bits per superbrick, whole words
     L.icp_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_mask = take((size_t)W * H);
@@ -510,7 +511,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     launch_vis_clear(B.dP, B.n, B.bl, s);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, s);
     mark(4);
-    launch_integrate(B.dP, B.bl, B.tt, B.wt, integ_grid(B), s);
+    launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), s);
     mark(5);
     const LevelWork lw = level_work(B.dWork, B.nWork, B.levels);
     launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, s);
@@ -837,6 +838,7 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.bocc = (uint32_t *)(m + L.bocc);
     P.bband = (uint32_t *)(m + L.bband);
     P.sband = (uint32_t *)(m + L.sband);
+    P.sdist = (uint8_t *)(m + L.sdist);
     for (int k = 0; k < 3; ++k) P.sdim[k] = L.sdim[k];
     P.sband_words = (int)L.nswords;
     P.icp_part = (double *)(m + L.icp_part);
@@ -1020,7 +1022,7 @@ vs_status vs_integrate(vs_session *s, vs_stream stream) {
     launch_fuse_ok(B.dP, 1, h);
     launch_vis_clear(B.dP, 1, B.bl, h);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, h);
-    launch_integrate(B.dP, B.bl, B.tt, B.wt, integ_grid(B), h);
+    launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
 }

@@ -27,7 +27,11 @@ constexpr int kIcpVals = 29;
 constexpr int kIcpGroup = 4;
 __host__ __device__ constexpr int icp_groups(int tiles) { return (tiles + kIcpGroup - 1) / kIcpGroup; }        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
 constexpr int kSuper = 32;          // superbrick edge (voxels) = 4 bricks
-constexpr int kSbandLdsWords = 1024; // superbrick band bits a raycast workgroup keeps in LDS (up to 1024^3 voxels)
+// Superbrick distance field: per superbrick the L-infinity distance (in
+// superbricks, capped at kDistMax) to the nearest band superbrick, 4 bits
+// each; sessions of up to kDistCells superbricks (1024^3 voxels) have one.
+constexpr int kDistMax = 15;
+constexpr int kDistCells = 32768;
 
 struct LevelDesc {
     int W, H;
@@ -49,6 +53,7 @@ struct alignas(16) SessState {
     float dist_cc;     // |cube centre - camera|
     float cos_max;     // raycast cone threshold
     int frames;        // frames fused so far
+    int sdist_dirty;   // a superbrick band bit was set since the distance field was built
     int active;        // the session has a frame in the current step
     int do_track;      // this frame runs the tracker
     int icp_failed;    // 0 running, else a vs_status code
@@ -101,6 +106,7 @@ struct alignas(16) SessParams {
     // whose bit is clear holds no band brick.
     uint32_t *bband;
     uint32_t *sband;
+    uint8_t *sdist;          // superbrick distance field, 2 superbricks per byte (low nibble first)
     int sdim[3];             // superbricks per axis
     int sband_words;         // 32-bit words of sband
     double *icp_part;        // ICP group partials (kIcpVals per group of level-0 tiles)

@@ -40,7 +40,8 @@ struct BrickList {
 };
 void launch_vis_clear(const SessParams *P, int nsess, BrickList bl, hipStream_t s);
 void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList bl, hipStream_t s);
-void launch_integrate(const SessParams *P, BrickList bl, int tsdf_type, int weight_type, int grid, hipStream_t s);
+void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_type, int weight_type, int grid,
+                      hipStream_t s);
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
                     hipStream_t s);
 // Several pyramid levels in one launch: segment k of the grid is level
@@ -57,6 +58,9 @@ struct LevelWork {
     }
 };
 LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels);
+// superbrick distance field of sessions P[0..nsess) (only those that gained
+// a band superbrick unless force)
+void launch_sdist(const SessParams *P, int nsess, int force, hipStream_t s);
 // Ray block of one wave in the raycast kernels (same results for each).
 enum RayBlock { kRay8x8 = 0, kRay16x4 = 1, kRay32x2 = 2 };
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,

@@ -278,7 +278,8 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
                 const uint32_t wo = ldg(P.bocc, b >> 5), wbb = ldg(P.bband, b >> 5), ws = ldg(P.sband, sb >> 5);
                 if (!(wo & bb)) atomicOr(P.bocc + (b >> 5), bb);
                 if (cb && !(wbb & bb)) atomicOr(P.bband + (b >> 5), bb);
-                if (cb && !(ws & sbb)) atomicOr(P.sband + (sb >> 5), sbb);
+                if (cb && !(ws & sbb) && !(atomicOr(P.sband + (sb >> 5), sbb) & sbb))
+                    stg(&P.st->sdist_dirty, 0, 1);  // a new band superbrick: rebuild the distance field
                 if (it.x != run_sess) {
                     if (run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
                     run_sess = it.x;
@@ -290,6 +291,81 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
         parity ^= 1;
     }
     if (t == 0 && run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
+}
+
+// ---------------------------------------------------------------------------
+// Superbrick distance field of the raycast's skip boxes: for every superbrick
+// the L-infinity distance, in superbricks, to the nearest band superbrick,
+// capped at kDistMax (0 = a band superbrick; cells outside the grid count as
+// free), so the (2d-1)^3 superbricks centred on one at distance d hold no
+// band brick. Separable: the distance along x within each row, then
+// d'(p) = min over |t| < kDistMax of max(|t|, d(p + t*e)) along y and then z.
+// One workgroup per session, nibble arrays in LDS; rebuilt only when a
+// session gained a band superbrick (or when forced).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int nib(const uint8_t *a, int c) { return (a[c >> 1] >> ((c & 1) * 4)) & 15; }
+
+// one separable pass along `axis` (stride `step` cells, extent `len`) from src to dst
+__device__ __forceinline__ void dist_pass(const uint8_t *src, uint8_t *dst, int n, int nx, int ny, int axis) {
+    for (int i = threadIdx.x; i < (n + 1) / 2; i += blockDim.x) {
+        uint8_t out = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = 2 * i + h;
+            if (c >= n) break;
+            const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+            const int pos = axis == 0 ? x : axis == 1 ? y : z;
+            const int len = axis == 0 ? nx : axis == 1 ? ny : n / (nx * ny);
+            const int step = axis == 0 ? 1 : axis == 1 ? nx : nx * ny;
+            int d = nib(src, c);
+            for (int t = 1; t < kDistMax && t < d; ++t) {
+                if (pos - t >= 0) d = min(d, max(t, nib(src, c - t * step)));
+                if (pos + t < len) d = min(d, max(t, nib(src, c + t * step)));
+            }
+            out |= (uint8_t)(d << (4 * h));
+        }
+        dst[i] = out;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sdist(const SessParams *__restrict__ SP, int nsess, int force) {
+    __shared__ uint8_t A[kDistCells / 2], B[kDistCells / 2];
+    const int s = blockIdx.x;
+    if (s >= nsess) return;
+    const SessParams &P = SP[s];
+    const int nx = P.sdim[0], ny = P.sdim[1], n = P.sdim[0] * P.sdim[1] * P.sdim[2];
+    if (n <= 0 || n > kDistCells) return;
+    if (!force && !ldg(&P.st->sdist_dirty, 0)) return;
+    for (int i = threadIdx.x; i < (n + 1) / 2; i += blockDim.x) {
+        uint8_t out = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = 2 * i + h;
+            const bool band = c < n && ((ldg(P.sband, c >> 5) >> (c & 31)) & 1u);
+            out |= (uint8_t)((band ? 0 : kDistMax) << (4 * h));
+        }
+        A[i] = out;
+    }
+    __syncthreads();
+    dist_pass(A, B, n, nx, ny, 0);
+    __syncthreads();
+    dist_pass(B, A, n, nx, ny, 1);
+    __syncthreads();
+    dist_pass(A, B, n, nx, ny, 2);
+    __syncthreads();
+    // whole words (the buffer holds (n + 7) / 8 of them; cells past n read 0)
+    uint32_t *dst = (uint32_t *)P.sdist;
+    for (int i = threadIdx.x; i < (n + 7) / 8; i += blockDim.x) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w |= (4 * i + k < (n + 1) / 2 ? (uint32_t)B[4 * i + k] : 0u) << (8 * k);
+        stg(dst, (size_t)i, w);
+    }
+    if (threadIdx.x == 0) stg(&P.st->sdist_dirty, 0, 0);
+}
+
+void launch_sdist(const SessParams *P, int nsess, int force, hipStream_t s) {
+    if (nsess > 0) hipLaunchKernelGGL(k_sdist, dim3(nsess), dim3(256), 0, s, P, nsess, force);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,28 +382,27 @@ struct Vol {
     const SessParams &P;
     const TV *F;
     const TW *Wt;
-    // superbrick band bits: the workgroup's LDS copy of kSbandLdsWords words.
-    // A session whose bitmap does not fit gets a copy of all ones (every
-    // superbrick counts as a band one and only the brick bits decide); the
-    // index is masked to the copy's size either way.
-    const VS_LDS uint32_t *sbits;
-    // band bits of the last brick looked up (a ray's consecutive taps mostly
-    // stay in one brick; the bitmaps are read-only during the launch), in one
-    // register: brick index in bits 0-29 (< 2^24 bricks), bit 30 = its
-    // superbrick is a band one, bit 31 = the brick is a band one
-    static constexpr unsigned kIdx = 0x3FFFFFFFu, kSuperBand = 1u << 30, kBrickBand = 1u << 31;
+    // superbrick distance field: the workgroup's LDS copy (4 bits per
+    // superbrick). A session without one gets zeros (every superbrick counts
+    // as a band one and only the brick bits decide); the index is masked to
+    // the copy's size either way.
+    const VS_LDS uint8_t *sdist;
+    // of the last brick looked up (a ray's consecutive taps mostly stay in
+    // one brick; the bitmaps are read-only during the launch), in one
+    // register: brick index in bits 0-24 (< 2^24 bricks, so the initial
+    // 2^25 - 1 matches none), its superbrick's distance in bits 25-28, bit 31
+    // = the brick is a band one
+    static constexpr unsigned kIdx = 0x01FFFFFFu, kBrickBand = 1u << 31;
     mutable unsigned cbf = kIdx;
     __device__ __forceinline__ int cb() const { return (int)(cbf & kIdx); }
-    __device__ __forceinline__ bool super_band() const { return cbf & kSuperBand; }
+    __device__ __forceinline__ int dist() const { return (int)((cbf >> 25) & 15u); }
     __device__ __forceinline__ bool band(int x, int y, int z) const {
         const unsigned b = (unsigned)brick_of(P, x, y, z);
         if (b != (cbf & kIdx)) {
-            unsigned f = b;
             const unsigned sb = (unsigned)super_of_brick(P, x >> 3, y >> 3, z >> 3);
-            if ((sbits[(sb >> 5) & (kSbandLdsWords - 1)] >> (sb & 31)) & 1u) {
-                f |= kSuperBand;
-                if ((ldg(P.bband, b >> 5) >> (b & 31)) & 1u) f |= kBrickBand;
-            }
+            const unsigned d = (sdist[(sb >> 1) & (kDistCells / 2 - 1)] >> ((sb & 1u) * 4u)) & 15u;
+            unsigned f = b | (d << 25);
+            if (d == 0u && ((ldg(P.bband, b >> 5) >> (b & 31)) & 1u)) f |= kBrickBand;
             cbf = f;
         }
         return cbf & kBrickBand;
@@ -376,9 +451,15 @@ struct Vol {
             return 0;
         }
         if (!band(x, y, z)) {
-            const int sh = super_band() ? 3 : 5;  // a brick, or a whole superbrick, without band bit
-            lo = make_int3((x >> sh) << sh, (y >> sh) << sh, (z >> sh) << sh);
-            size = 1 << sh;
+            const int d = dist();
+            if (d == 0) {  // a brick without band bit in a band superbrick
+                lo = make_int3(x & ~7, y & ~7, z & ~7);
+                size = kBrick;
+            } else {  // the (2d-1)^3 superbricks centred here hold no band brick
+                const int r = d - 1;
+                lo = make_int3(((x >> 5) - r) * kSuper, ((y >> 5) - r) * kSuper, ((z >> 5) - r) * kSuper);
+                size = (2 * r + 1) * kSuper;
+            }
             return 0;
         }
         return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
@@ -705,17 +786,19 @@ __device__ __forceinline__ void ray_pixel(int t, int &px, int &py) {
 template <class TV, class TW, bool STATS, int RB>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_raycast(
     const SessParams *__restrict__ SP, LevelWork lw, unsigned long long *stats) {
-    __shared__ uint32_t s_sband[kSbandLdsWords];
+    __shared__ uint32_t s_dist[kDistCells / 8];
     int level;
     const WorkItem wi = level_item(lw, level);
     const SessParams &P = SP[wi.sess];
     const SessState *st = P.st;
     const LevelDesc &L = P.lev[level];
     if (!st->fuse_ok || level >= P.levels || tile_out(L, wi.tile)) return;
-    if (P.sband_words <= kSbandLdsWords) {
-        for (int i = threadIdx.x; i < P.sband_words; i += kTileThreads) s_sband[i] = ldg(P.sband, i);
+    const int ncell = P.sdim[0] * P.sdim[1] * P.sdim[2];
+    if (ncell <= kDistCells) {
+        const uint32_t *src = (const uint32_t *)P.sdist;
+        for (int i = threadIdx.x; i < (ncell + 7) / 8; i += kTileThreads) s_dist[i] = ldg(src, i);
     } else {
-        for (int i = threadIdx.x; i < kSbandLdsWords; i += kTileThreads) s_sband[i] = 0xFFFFFFFFu;
+        for (int i = threadIdx.x; i < kDistCells / 8; i += kTileThreads) s_dist[i] = 0u;
     }
     __syncthreads();
     int px, py;
@@ -734,7 +817,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     C.step_scale = P.step_scale;
     C.total_max = P.total_max;
     C.max_iter = P.max_iter;
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint32_t *)s_sband};
+    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint8_t *)s_dist};
     unsigned n_near = 0, n_tri = 0;
     f3 p;
     const M34 Tc = ld_m34(st->T_cg_f);
@@ -997,8 +1080,9 @@ template <class TV, class TW>
 static void integrate_t(const SessParams *P, BrickList bl, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_integrate<TV, TW>), dim3(grid), dim3(kTileThreads), 0, s, P, bl);
 }
-void launch_integrate(const SessParams *P, BrickList bl, int tt, int wt, int grid, hipStream_t s) {
+void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tt, int wt, int grid, hipStream_t s) {
     VS_DISPATCH(tt, wt, integrate_t, P, bl, grid, s);
+    launch_sdist(P, nsess, 0, s);
 }
 
 template <class TV, class TW>
@@ -1079,6 +1163,7 @@ static void init_t(const SessParams *P, size_t nvox, uint32_t nwords, hipStream_
 }
 void launch_volume_init(const SessParams *P, size_t nvox, int tt, int wt, uint32_t nwords, hipStream_t s) {
     VS_DISPATCH(tt, wt, init_t, P, nvox, nwords, s);
+    launch_sdist(P, 1, 1, s);
 }
 template <class TV, class TW>
 static void to_lin_t(const SessParams *P, size_t nvox, float *F, float *Wt, hipStream_t s) {
@@ -1104,6 +1189,7 @@ void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tt, int
     if (g <= 0) return;
     hipLaunchKernelGGL(k_sband_clear, dim3(16), dim3(256), 0, s, P);
     VS_DISPATCH(tt, wt, occupancy_t, P, nbricks, g, s);
+    launch_sdist(P, 1, 1, s);
 }
 
 template <class TV, class TW>
