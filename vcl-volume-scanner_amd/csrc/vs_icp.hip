@@ -156,48 +156,55 @@ __device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &ou
     return ok;
 }
 
+// The summand of pixel (x, y) as a branch-free sequence: every test of the
+// reference is evaluated (same FP32 expressions, so the same decisions) and
+// folded into `ok`; the model taps use coordinates clamped into the map, so
+// every load is in range whatever the tests say, and a rejected pixel's
+// Jacobian and residual are zero.
 __device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, int y, const float *T21, float A[6],
                                             float &b) {
     const LevelDesc &L = P.lev[l];
     const int W = L.W, H = L.H;
     const float td1 = ldg(P.filt[l], (size_t)y * W + x);
-    if (td1 < 1.0e-6f) return false;
     const f3 p = transform(T21, back_project((float)x, (float)y, L.K, td1));
     float u, v;
     project(p, L.K, u, v);
-    if (!(u >= 0.0f && u <= float(W - 1) && v >= 0.0f && v <= float(H - 1))) return false;
-    const int x0 = (int)u, y0 = (int)v;
-    const float rx = u - (float)x0, ry = v - (float)y0;
+    const bool inside = u >= 0.0f && u <= float(W - 1) && v >= 0.0f && v <= float(H - 1);
+    bool ok = !(td1 < 1.0e-6f) && inside;
+    const float uc = inside ? u : 0.0f, vc = inside ? v : 0.0f;
+    const int x0 = (int)uc, y0 = (int)vc;
+    const float rx = uc - (float)x0, ry = vc - (float)y0;
     const size_t i0 = (size_t)y0 * W + x0, di = rx > 0.0f ? 1 : 0, dj = ry > 0.0f ? (size_t)W : 0;
     const Quad qv = load_quad(P.mvtx[l], i0, di, dj);
     const Quad qn = load_quad(P.mnrm[l], i0, di, dj);
     f3 v0, n;
-    if (!bilerp(qv, rx, ry, v0)) return false;
+    ok &= bilerp(qv, rx, ry, v0);
     const float dx = v0.x - p.x, dy = v0.y - p.y, dz = v0.z - p.z;
-    if (dx * dx + dy * dy + dz * dz > P.icp_dist2) return false;
-    if (!bilerp(qn, rx, ry, n)) return false;
+    ok &= !(dx * dx + dy * dy + dz * dz > P.icp_dist2);
+    ok &= bilerp(qn, rx, ry, n);
     n = normalised(n);
-    A[0] = p.z * n.y + (-p.y) * n.z;
-    A[1] = (-p.z) * n.x + p.x * n.z;
-    A[2] = p.y * n.x + (-p.x) * n.y;
-    A[3] = -n.x;
-    A[4] = -n.y;
-    A[5] = -n.z;
-    b = n.x * dx + n.y * dy + n.z * dz;
-    return true;
+    A[0] = ok ? p.z * n.y + (-p.y) * n.z : 0.0f;
+    A[1] = ok ? (-p.z) * n.x + p.x * n.z : 0.0f;
+    A[2] = ok ? p.y * n.x + (-p.x) * n.y : 0.0f;
+    A[3] = ok ? -n.x : 0.0f;
+    A[4] = ok ? -n.y : 0.0f;
+    A[5] = ok ? -n.z : 0.0f;
+    b = ok ? n.x * dx + n.y * dy + n.z * dz : 0.0f;
+    return ok;
 }
 
-// 29 sums of one pixel: JtJ upper triangle (21), Jtr (6), count, r^2.
-__device__ __forceinline__ void pixel_terms(bool ok, const float A[6], float b, float v[kIcpVals]) {
+// Adds one pixel's 29 terms (JtJ upper triangle, Jtr, count, r^2) to the
+// thread's FP32 sums; a rejected pixel has A = 0, b = 0 and adds zeros.
+__device__ __forceinline__ void add_terms(bool ok, const float A[6], float b, float acc[kIcpVals]) {
     int k = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
-        for (int j = i; j < 6; ++j) v[k++] = ok ? A[i] * A[j] : 0.0f;
+        for (int j = i; j < 6; ++j, ++k) acc[k] = fmaf(A[i], A[j], acc[k]);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) v[k++] = ok ? A[i] * b : 0.0f;
-    v[k++] = ok ? 1.0f : 0.0f;
-    v[k++] = ok ? b * b : 0.0f;
+    for (int i = 0; i < 6; ++i, ++k) acc[k] = fmaf(A[i], b, acc[k]);
+    acc[k++] += ok ? 1.0f : 0.0f;
+    acc[k] = fmaf(b, b, acc[k]);
 }
 
 // Block sum of the 29 terms: FP32 within each wave (64 pixels), FP64 across
@@ -253,13 +260,11 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
         const int tile = t0 + g;
         const int x = (tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
         const int y = (tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
-        float A[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
-        bool ok = false;
-        if (x < L.W && y < L.H) ok = icp_summand(P, level, x, y, T21s, A, b);
-        float v[kIcpVals];
-        pixel_terms(ok, A, b, v);
-#pragma unroll
-        for (int k = 0; k < kIcpVals; ++k) acc[k] += v[k];
+        if (x < L.W && y < L.H) {
+            float A[6], b;
+            const bool ok = icp_summand(P, level, x, y, T21s, A, b);
+            add_terms(ok, A, b, acc);
+        }
     }
     block_terms(acc, red64);
     if (threadIdx.x < kIcpVals) stg(P.icp_part, (size_t)wi.tile * kIcpVals + threadIdx.x, red64[threadIdx.x]);
@@ -444,7 +449,9 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_system(const SessParams *_
         if (mask) mask[(size_t)y * L.W + x] = ok ? 1 : 0;
     }
     float v[kIcpVals];
-    pixel_terms(ok, A, b, v);
+#pragma unroll
+    for (int k = 0; k < kIcpVals; ++k) v[k] = 0.0f;
+    add_terms(ok, A, b, v);
     block_terms(v, red64);
     if (threadIdx.x < kIcpVals) out_part[(size_t)wi.tile * kIcpVals + threadIdx.x] = red64[threadIdx.x];
 }
