@@ -207,21 +207,36 @@ __device__ __forceinline__ void add_terms(bool ok, const float A[6], float b, fl
     acc[k] = fmaf(b, b, acc[k]);
 }
 
-// Block sum of the 29 terms: FP32 within each wave (64 pixels), FP64 across
-// the 4 waves. Result in red64 (shared), valid after the call for all threads.
+// Block sum of the 29 terms through LDS: every thread stores its 29 FP32
+// sums (term-major rows of 257 words, so that the stores of a wave and the
+// reads below spread over the LDS banks), 8 threads per term each add a fixed
+// chunk of 32 threads' values in FP32 (in a fixed order), and the 8 chunk sums
+// of a term are added in FP64 in chunk order. Result in red64 (shared), valid
+// after the call for all threads. (29 wave-wide DPP trees cost ~11 VALU
+// instructions per term and wave; this costs ~4 per term and thread.)
+constexpr int kTermRow = kTileThreads + 1;
+constexpr int kChunks = 8, kChunk = kTileThreads / kChunks;
 __device__ __forceinline__ void block_terms(float v[kIcpVals], double *red64) {
-    __shared__ float wsum[kTileThreads / 64][kIcpVals];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ float rows[kIcpVals * kTermRow];
+    __shared__ double part[kIcpVals][kChunks];
+    const int t = threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < kIcpVals; ++k) {
-        float s = wave_sum_dpp(v[k]);
-        if (lane == 0) wsum[wv][k] = s;
+    for (int k = 0; k < kIcpVals; ++k) rows[k * kTermRow + t] = v[k];
+    __syncthreads();
+    if (t < kIcpVals * kChunks) {
+        const int k = t / kChunks, c = t % kChunks;
+        const float *row = rows + k * kTermRow + c * kChunk;
+        float s = 0.0f;
+#pragma unroll 8
+        for (int i = 0; i < kChunk; ++i) s += row[(i + 4 * c) & (kChunk - 1)];
+        part[k][c] = (double)s;
     }
     __syncthreads();
-    if (threadIdx.x < kIcpVals) {
+    if (t < kIcpVals) {
         double s = 0.0;
-        for (int w = 0; w < kTileThreads / 64; ++w) s += (double)wsum[w][threadIdx.x];
-        red64[threadIdx.x] = s;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) s += part[t][c];
+        red64[t] = s;
     }
     __syncthreads();
 }

@@ -328,7 +328,7 @@ __device__ __forceinline__ void dist_pass(const uint8_t *src, uint8_t *dst, int 
     }
 }
 
-__global__ __launch_bounds__(256) void k_sdist(const SessParams *__restrict__ SP, int nsess, int force) {
+__global__ __launch_bounds__(1024) void k_sdist(const SessParams *__restrict__ SP, int nsess, int force) {
     __shared__ uint8_t A[kDistCells / 2], B[kDistCells / 2];
     const int s = blockIdx.x;
     if (s >= nsess) return;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_sdist(const SessParams *__restrict__ SP
 }
 
 void launch_sdist(const SessParams *P, int nsess, int force, hipStream_t s) {
-    if (nsess > 0) hipLaunchKernelGGL(k_sdist, dim3(nsess), dim3(256), 0, s, P, nsess, force);
+    if (nsess > 0) hipLaunchKernelGGL(k_sdist, dim3(nsess), dim3(1024), 0, s, P, nsess, force);
 }
 
 // ---------------------------------------------------------------------------
