@@ -56,6 +56,34 @@ def test_batch_equals_alone(ctx, types):
             s.close()
 
 
+def test_split_batch_equals_alone(ctx):
+    """A batch of 16 or more sessions runs in parts, independent branches
+    of the captured step graph; every session still matches its lone run bit
+    for bit."""
+    print("case: 18-session batch (graph branches) vs alone, 4 frames", flush=True)
+    specs = [(3, 2000 + i, 160 if i % 3 else 320, 120 if i % 3 else 240, 64 if i % 2 else 96) for i in range(18)]
+    made = _sessions(ctx, vscan.VS_TSDF_F16, vscan.VS_WEIGHT_U8, specs)
+    alone = [vscan.Session(ctx, d, T_gc0=st.pose(0)) for st, d in made]
+    mixed = [vscan.Session(ctx, d, T_gc0=st.pose(0)) for st, d in made]
+    try:
+        for fr in range(4):
+            frames = [st.frame(fr) for st, _ in made]
+            for s, f in zip(alone, frames):
+                s.fuse(f)
+            ctx.fuse_batch(mixed, frames)
+        for a, m in zip(alone, mixed):
+            assert a.status() == m.status()
+            assert np.array_equal(a.pose(), m.pose())
+            Fa, Wa, _ = a.volume()
+            Fm, Wm, _ = m.volume()
+            assert np.array_equal(Fa, Fm) and np.array_equal(Wa, Wm)
+            for l in range(3):
+                assert all(np.array_equal(x, y) for x, y in zip(a.maps(l), m.maps(l)))
+    finally:
+        for s in alone + mixed:
+            s.close()
+
+
 def test_fp16_form_tracks_fp64_oracle(ctx):
     print("case: FP16 + u8 volume vs FP64 oracle, 320x240 / 128^3, 20 frames", flush=True)
     st = vscan.SceneStream(4, seed=1)

@@ -258,29 +258,4 @@ __device__ __forceinline__ float ld_w(const uint8_t *p, size_t i) { return (floa
 __device__ __forceinline__ void st_w(float *p, size_t i, float v) { stg(p, i, v); }
 __device__ __forceinline__ void st_w(uint8_t *p, size_t i, float v) { stg(p, i, (uint8_t)v); }
 
-// Wave64 sum via cross-lane shuffles (ds_bpermute: an LDS round trip per step).
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Wave64 sum with DPP lane moves on the VALU (no LDS): xor-1 and xor-2 within
-// quads, rotations by 4 and 8 within rows of 16, then row 0 -> 1 and row 2 ->
-// 3 (row_bcast:15), rows 0-1 -> 2-3 (row_bcast:31); lane 63 holds the total,
-// returned to every lane.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_f(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
-}
-__device__ __forceinline__ float wave_sum_dpp(float v) {
-    v += dpp_f<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-    v += dpp_f<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-    v += dpp_f<0x124, 0xF>(v);  // row_ror:4
-    v += dpp_f<0x128, 0xF>(v);  // row_ror:8
-    v += dpp_f<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_f<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
 }  // namespace vs
