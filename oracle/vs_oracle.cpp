@@ -26,8 +26,12 @@
 //  Q12 the raycast cone test compares cosines: the ray is culled when
 //      dot(rd, n_cc) < sqrt(1 - (r/d)^2) (no culling when r >= d), where the
 //      reference compares acosf(dot) > asin(r/d).
-//  Q13 the march is capped at 4 * total_step_max / min(step_scale, 1) + 64
-//      iterations (a fine-search ray can oscillate forever in the reference).
+//  Q13 the march is capped at total_step_max / min(step_scale, 1) + 64
+//      iterations: the most steps a ray can take while every step moves it
+//      forward by at least min(step_scale, 1) voxels, plus 64. Only a
+//      fine-search ray that keeps stepping back (a -1.0 voxel behind the
+//      surface) reaches it; the reference loop never ends for such a ray
+//      (total_step sums signed step lengths, tsdf_renderer.cu:544-560).
 //  Q14 bilinear taps that the reference skips (zero residual) contribute 0.
 //  Q15 a Cholesky pivot that is not >= 1e-8 (including NaN) fails the solve.
 //  Q16 Vector3f::toUChar() (ITMMath, not vendored) = ROUND half away from zero,
@@ -819,7 +823,7 @@ RayParams ray_params(const Vol &V, const float *K, const float *Tgc, const float
     P.cos_max = (s < 1.0f) ? sqrtf(1.0f - s * s) : -2.0f;
     P.step_scale = 0.9f * V.mu * V.vox_inv;
     P.total_max = 2.0f * P.r_cube * V.vox_inv;
-    P.max_iter = (int)(4.0f * P.total_max / std::min(P.step_scale, 1.0f)) + 64;
+    P.max_iter = (int)(P.total_max / std::min(P.step_scale, 1.0f)) + 64;
     return P;
 }
 

@@ -860,7 +860,12 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     }
     P.step_scale = 0.9f * P.mu * P.vox_inv;
     P.total_max = 2.0f * P.r_cube * P.vox_inv;
-    P.max_iter = (int)(4.0f * P.total_max / std::min(P.step_scale, 1.0f)) + 64;
+    // the march's iteration cap: the most steps of a ray that only moves
+    // forward (by at least min(step_scale, 1) voxels a step) before its
+    // distance cap, plus 64; only a ray that keeps stepping back behind a
+    // surface reaches it, and the reference loop would not end for such a
+    // ray (tsdf_renderer.cu:544-560 sums signed step lengths)
+    P.max_iter = (int)(P.total_max / std::min(P.step_scale, 1.0f)) + 64;
     // bilateral spatial weights exactly as updateGaussian (img_proc.cu:275-286)
     for (int i = 0; i < 2 * d.bil_radius + 1; ++i) {
         float x = (float)(i - d.bil_radius);

@@ -502,7 +502,7 @@ struct RayConst {
 // positions below 2^13 voxels (< 0.02 voxel) and the error of the reciprocal
 // estimates (relative 2^-22 of at most 64 steps), so each skipped step's tap
 // would have read 1.0 too. The skipped additions are performed one by
-// one (a closed form per binade, vs_fprep.hpp, costs more VALU than the
+// one (a closed form per binade costs more VALU than the
 // handful of additions a box skips on average).
 constexpr int kSkipMax = 64;
 constexpr float kSkipMargin = 0.25f;
