@@ -233,3 +233,58 @@ def _track_parity(sess, stream):
     # and both land near the ground truth of frame 11
     gt_t, gt_a = orc.pose_error(T_or, stream.pose(11))
     assert gt_t < 0.01 and gt_a < 1.0, (gt_t, gt_a)
+
+
+def test_noncubic_grid_odd_frame(ctx):
+    """A grid of 120 x 104 x 72 voxels (sides not multiples of the 32-voxel
+    superbrick, so the last superbrick of each axis is partly outside) and a
+    200x150 frame (levels 100x75 and 50x37, odd halvings): integrate's brick
+    mask, visible and updated counts and weights bit-exact and the TSDF within
+    1e-4 of the oracle; on the oracle's volume, the raycast's hit masks
+    bit-exact and points within 0.5 voxel at every level. Cubic grids and
+    even frames cannot catch an axis mix-up in the brick or superbrick
+    indexing."""
+    print("case: non-cubic 120x104x72 grid, 200x150 frame", flush=True)
+    st = vscan.SceneStream(0, seed=2, width=200, height=150)
+    d = vscan.make_desc(W0, K=st.K, W=200, H=150)
+    d.vol_dim[0], d.vol_dim[1], d.vol_dim[2] = 120, 104, 72
+    s = vscan.Session(ctx, d, T_gc0=st.pose(0))
+    try:
+        vol = orc.volume(d)
+        N = 120 * 104 * 72
+        assert s.nvox == N
+        F = np.ones(N)
+        Wt = np.zeros(N)
+        K = orc.K_levels(d)
+        for fr in (0, 4, 8):
+            T = st.pose(fr)
+            mm = st.frame(fr)
+            s.set_pose(T)
+            s.set_depth(mm)
+            s.integrate()
+            Tg, Tc = f32pose(T)
+            bm, upd, counts = orc.integrate(mm, K[0], Tg, Tc, vol, F, Wt)
+            gF, gW, gbm = s.volume()
+            assert gF.shape == (72, 104, 120)
+            assert np.array_equal(gbm, bm), f"brick mask differs at frame {fr}"
+            assert s.integrate_counts() == (counts[0], counts[1]), (s.integrate_counts(), counts)
+            assert np.array_equal(gW.reshape(-1), Wt.astype(np.float32)), "weights differ"
+            assert np.abs(gF.reshape(-1).astype(np.float64) - F).max() <= 1e-4
+        assert counts[1] > 10000
+        F32, W32 = F.astype(np.float32), Wt.astype(np.float32)
+        s.set_volume(F32, W32)
+        T = st.pose(10)
+        s.set_pose(T)
+        Tg, Tc = f32pose(T)
+        for l in range(3):
+            s.raycast(l)
+            _, gv, _, _ = s.maps(l)
+            h, w = s.level_shape(l)
+            assert (w, h) == ((200, 150), (100, 75), (50, 37))[l]
+            _, rv, _, _, _ = orc.raycast(F32, W32, vol, w, h, K[l], Tg, Tc)
+            gm, rm = gv[..., 0] != -100, rv[..., 0] != -100
+            assert np.array_equal(gm, rm), f"level {l}: {int(np.sum(gm != rm))} rays differ in hit/miss"
+            assert gm.sum() > 0.1 * gm.size
+            assert np.abs(gv - rv)[rm].max() <= 0.5 * d.voxel
+    finally:
+        s.close()
