@@ -236,7 +236,7 @@ def _track_parity(sess, stream):
 
 
 def test_noncubic_grid_odd_frame(ctx):
-    """A grid of 120 x 104 x 72 voxels (sides not multiples of the 32-voxel
+    """A grid of 120 x 104 x 72 voxels (sides not multiples of the 16-voxel
     superbrick, so the last superbrick of each axis is partly outside) and a
     200x150 frame (levels 100x75 and 50x37, odd halvings): integrate's brick
     mask, visible and updated counts and weights bit-exact and the TSDF within

@@ -54,7 +54,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, bband, sband, sdist, icp_part, diag_part, diag_mask, lists, params, stats, keys;
+    size_t tsdf, weight, bvis, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t total;
     size_t nvox;
@@ -90,7 +90,7 @@ Layout make_layout(const vs_session_desc &d) {
     L.nvox = (size_t)d.vol_dim[0] * d.vol_dim[1] * d.vol_dim[2];
     L.nbricks = (uint32_t)((d.vol_dim[0] / 8) * (d.vol_dim[1] / 8) * (d.vol_dim[2] / 8));
     L.nwords = (L.nbricks + 31) / 32;
-    for (int k = 0; k < 3; ++k) L.sdim[k] = (d.vol_dim[k] / 8 + 3) / 4;
+    for (int k = 0; k < 3; ++k) L.sdim[k] = (d.vol_dim[k] / kBrick + (1 << kSuperShift) - 1) >> kSuperShift;
     L.nsuper = (uint32_t)(L.sdim[0] * L.sdim[1] * L.sdim[2]);
     L.nswords = (L.nsuper + 31) / 32;
     const size_t sv = d.tsdf_type == VS_TSDF_F16 ? 2 : 4, sw = d.weight_type == VS_WEIGHT_U8 ? 1 : 4;
@@ -113,6 +113,7 @@ Layout make_layout(const vs_session_desc &d) {
     L.bband = take((size_t)L.nwords * 4);
     L.sband = take((size_t)L.nswords * 4);
     L.sdist = take(((size_t)L.nsuper + 7) / 8 * 4);  // 4 bits per superbrick, whole words
+    L.sdtmp = take(L.nsuper);
     L.icp_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_part = take((size_t)L.tiles[0] * kIcpVals * 8);
     L.diag_mask = take((size_t)W * H);
@@ -220,6 +221,7 @@ struct TuneState {
 struct Batch {
     int n = 0, tt = 0, wt = 0, levels = 0;
     int max_iters[kMaxLevels] = {0, 0, 0};
+    int sd_slices = 1;         // most superbricks along z of a session (distance field passes)
     SessParams *dP = nullptr;  // device array
     WorkItem *dWork[kMaxLevels] = {nullptr, nullptr, nullptr};   // (slot, tile)
     int nWork[kMaxLevels] = {0, 0, 0};
@@ -349,6 +351,7 @@ struct ListSet {
         B.bl.capacity = bricks;
         HIP_TRY(hipMemset(B.bl.count, 0, 16));
         B.levels = levels;
+        B.sd_slices = (max_dim / kBrick + (1 << kSuperShift) - 1) >> kSuperShift;
         for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = max_iters[l];
         B.bricks = bricks;
         B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(bricks, (uint32_t)integ_max));
@@ -524,7 +527,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     launch_vis_clear(B.dP, B.n, B.bl, s);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, s);
     mark(4);
-    launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), s);
+    launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), B.sd_slices, s);
     mark(5);
     const LevelWork lw = level_work(B.dWork, B.nWork, B.levels);
     launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, s);
@@ -887,6 +890,7 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.bband = (uint32_t *)(m + L.bband);
     P.sband = (uint32_t *)(m + L.sband);
     P.sdist = (uint8_t *)(m + L.sdist);
+    P.sdtmp = (uint8_t *)(m + L.sdtmp);
     for (int k = 0; k < 3; ++k) P.sdim[k] = L.sdim[k];
     P.sband_words = (int)L.nswords;
     P.icp_part = (double *)(m + L.icp_part);
@@ -957,7 +961,7 @@ vs_status vs_session_destroy(vs_session *s) {
 vs_status vs_session_reset(vs_session *s, const double T_gc0[16], vs_stream stream) {
     if (!s) return fail(VS_EINVAL, "null session");
     hipStream_t h = pick(s, stream);
-    launch_volume_init(s->self.dP, s->L.nvox, s->d.tsdf_type, s->d.weight_type, s->L.nwords, h);
+    launch_volume_init(s->self.dP, s->L.nvox, s->d.tsdf_type, s->d.weight_type, s->L.nwords, s->L.sdim[2], h);
     SessState init{};
     for (int k = 0; k < 16; ++k) init.T_gc[k] = T_gc0 ? T_gc0[k] : ((k % 5) == 0 ? 1.0 : 0.0);
     init.conv_level = -1;
@@ -1070,7 +1074,7 @@ vs_status vs_integrate(vs_session *s, vs_stream stream) {
     launch_fuse_ok(B.dP, 1, h);
     launch_vis_clear(B.dP, 1, B.bl, h);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, h);
-    launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), h);
+    launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), B.sd_slices, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
 }
@@ -1460,7 +1464,7 @@ vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_sess
                 launch_brick_pack(s->self.dP, (const uint32_t *)dev, (unsigned)n, (int)h.value_bytes,
                                   (int)h.weight_bytes, (int)h.color_bytes, (uint8_t *)dev + align_up(n * 4), 1,
                                   s->stream);
-                launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->stream);
+                launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->L.sdim[2], s->stream);
                 e = hipStreamSynchronize(s->stream);
             }
             hipFree(dev);
@@ -1545,7 +1549,7 @@ vs_status vs_volume_upload(vs_session *s, const float *tsdf, const float *weight
     if (e == hipSuccess) e = hipMemcpy(tmp + N, weight, N * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         launch_volume_from_linear(s->self.dP, N, s->d.tsdf_type, s->d.weight_type, tmp, tmp + N, s->stream);
-        launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->stream);
+        launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->L.sdim[2], s->stream);
         e = hipStreamSynchronize(s->stream);
     }
     hipFree(tmp);

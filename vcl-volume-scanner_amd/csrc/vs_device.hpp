@@ -26,10 +26,16 @@ constexpr int kIcpVals = 29;
 // tiles, so the sums do not depend on the batch)
 constexpr int kIcpGroup = 4;
 __host__ __device__ constexpr int icp_groups(int tiles) { return (tiles + kIcpGroup - 1) / kIcpGroup; }        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
-constexpr int kSuper = 32;          // superbrick edge (voxels) = 4 bricks
+// Superbricks: cubes of 2^3 bricks (16^3 voxels). The raycast's distance
+// field (below) is kept per superbrick and copied to each raycast
+// workgroup's LDS, whose 16 KB hold the 32^3 superbricks of a 512^3 grid;
+// smaller cells bound the band regions more tightly (a band superbrick of
+// 32^3 voxels left the march ~6 brick-sized boxes per ray inside it).
+constexpr int kSuperShift = 1;                   // log2(bricks per superbrick edge)
+constexpr int kSuper = kBrick << kSuperShift;    // superbrick edge (voxels)
 // Superbrick distance field: per superbrick the L-infinity distance (in
 // superbricks, capped at kDistMax) to the nearest band superbrick, 4 bits
-// each; sessions of up to kDistCells superbricks (1024^3 voxels) have one.
+// each; sessions of up to kDistCells superbricks (512^3 voxels) have one.
 constexpr int kDistMax = 15;
 constexpr int kDistCells = 32768;
 
@@ -102,11 +108,12 @@ struct alignas(16) SessParams {
     uint32_t *bocc;          // bricks holding any weight > 0 (monotone)
     // Band bits (set-only, so a superset): a brick whose bit is clear holds
     // only voxels of weight 0 or value exactly 1.0, which the raycast's march
-    // steps through like empty space; a superbrick (4^3 bricks = 32^3 voxels)
+    // steps through like empty space; a superbrick (2^3 bricks = 16^3 voxels)
     // whose bit is clear holds no band brick.
     uint32_t *bband;
     uint32_t *sband;
     uint8_t *sdist;          // superbrick distance field, 2 superbricks per byte (low nibble first)
+    uint8_t *sdtmp;          // its rebuild's scratch: one byte per superbrick
     int sdim[3];             // superbricks per axis
     int sband_words;         // 32-bit words of sband
     double *icp_part;        // ICP group partials (kIcpVals per group of level-0 tiles)
@@ -241,7 +248,7 @@ __device__ __forceinline__ int brick_of(const SessParams &P, int x, int y, int z
 }
 // superbrick of brick (bx, by, bz)
 __device__ __forceinline__ int super_of_brick(const SessParams &P, int bx, int by, int bz) {
-    return ((bz >> 2) * P.sdim[1] + (by >> 2)) * P.sdim[0] + (bx >> 2);
+    return ((bz >> kSuperShift) * P.sdim[1] + (by >> kSuperShift)) * P.sdim[0] + (bx >> kSuperShift);
 }
 
 // Storage conversions for the two volume forms (volume planes are in HBM).

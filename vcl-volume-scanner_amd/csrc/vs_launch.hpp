@@ -40,8 +40,9 @@ struct BrickList {
 };
 void launch_vis_clear(const SessParams *P, int nsess, BrickList bl, hipStream_t s);
 void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList bl, hipStream_t s);
+// sd_slices: the most superbricks along z of any session (distance field)
 void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_type, int weight_type, int grid,
-                      hipStream_t s);
+                      int sd_slices, hipStream_t s);
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
                     hipStream_t s);
 // Several pyramid levels in one launch: segment k of the grid is level
@@ -59,8 +60,8 @@ struct LevelWork {
 };
 LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels);
 // superbrick distance field of sessions P[0..nsess) (only those that gained
-// a band superbrick unless force)
-void launch_sdist(const SessParams *P, int nsess, int force, hipStream_t s);
+// a band superbrick unless force); sd_slices >= every session's sdim[2]
+void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipStream_t s);
 // Ray block of one wave in the raycast kernels (same results for each).
 enum RayBlock { kRay8x8 = 0, kRay16x4 = 1, kRay32x2 = 2 };
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
@@ -69,12 +70,13 @@ void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tsdf_typ
                           unsigned long long *stats, int ray_block, hipStream_t s);
 // volume maintenance (single session, P points at its parameters)
 void launch_volume_init(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, uint32_t nwords,
-                        hipStream_t s);
+                        int sd_slices, hipStream_t s);
 void launch_volume_to_linear(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, float *F, float *Wt,
                              hipStream_t s);
 void launch_volume_from_linear(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, const float *F,
                                const float *Wt, hipStream_t s);
-void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tsdf_type, int weight_type, hipStream_t s);
+void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tsdf_type, int weight_type, int sd_slices,
+                              hipStream_t s);
 void launch_color_to_linear(const SessParams *P, size_t nvox, uint8_t *rgb, hipStream_t s);
 // input pre-steps (vs_pre.hip); is_float selects float metres, else uint16 mm
 void launch_depth_reproject(int is_float, const void *src, int sw, int sh, const float Ks[4], const float Kd[4],

@@ -300,72 +300,79 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
 // free), so the (2d-1)^3 superbricks centred on one at distance d hold no
 // band brick. Separable: the distance along x within each row, then
 // d'(p) = min over |t| < kDistMax of max(|t|, d(p + t*e)) along y and then z.
-// One workgroup per session, nibble arrays in LDS; rebuilt only when a
-// session gained a band superbrick (or when forced).
+// Rebuilt only for sessions that gained a band superbrick (or when forced),
+// over many workgroups: pass 1, one per (session, z-slice of superbricks),
+// does x and y in LDS and leaves one byte per superbrick in the session's
+// scratch; pass 2, one per (session, 2048 superbricks), does z from the
+// scratch and packs two cells per byte of the field (byte-aligned chunks, so
+// no two workgroups write one byte); pass 3 clears the rebuild flags, which
+// every workgroup of passes 1 and 2 has read by then.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int nib(const uint8_t *a, int c) { return (a[c >> 1] >> ((c & 1) * 4)) & 15; }
+constexpr int kSdistThreads = 1024, kSdistChunk = 2 * kSdistThreads;
 
-// one separable pass along `axis` (stride `step` cells, extent `len`) from src to dst
-__device__ __forceinline__ void dist_pass(const uint8_t *src, uint8_t *dst, int n, int nx, int ny, int axis) {
-    for (int i = threadIdx.x; i < (n + 1) / 2; i += blockDim.x) {
-        uint8_t out = 0;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = 2 * i + h;
-            if (c >= n) break;
-            const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
-            const int pos = axis == 0 ? x : axis == 1 ? y : z;
-            const int len = axis == 0 ? nx : axis == 1 ? ny : n / (nx * ny);
-            const int step = axis == 0 ? 1 : axis == 1 ? nx : nx * ny;
-            int d = nib(src, c);
-            for (int t = 1; t < kDistMax && t < d; ++t) {
-                if (pos - t >= 0) d = min(d, max(t, nib(src, c - t * step)));
-                if (pos + t < len) d = min(d, max(t, nib(src, c + t * step)));
-            }
-            out |= (uint8_t)(d << (4 * h));
-        }
-        dst[i] = out;
-    }
+__device__ __forceinline__ bool sdist_todo(const SessParams &P, int force) {
+    const int n = P.sdim[0] * P.sdim[1] * P.sdim[2];
+    return n > 0 && n <= kDistCells && (force || ldg(&P.st->sdist_dirty, 0));
 }
 
-__global__ __launch_bounds__(1024) void k_sdist(const SessParams *__restrict__ SP, int nsess, int force) {
-    __shared__ uint8_t A[kDistCells / 2], B[kDistCells / 2];
-    const int s = blockIdx.x;
-    if (s >= nsess) return;
-    const SessParams &P = SP[s];
-    const int nx = P.sdim[0], ny = P.sdim[1], n = P.sdim[0] * P.sdim[1] * P.sdim[2];
-    if (n <= 0 || n > kDistCells) return;
-    if (!force && !ldg(&P.st->sdist_dirty, 0)) return;
-    for (int i = threadIdx.x; i < (n + 1) / 2; i += blockDim.x) {
-        uint8_t out = 0;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = 2 * i + h;
-            const bool band = c < n && ((ldg(P.sband, c >> 5) >> (c & 31)) & 1u);
-            out |= (uint8_t)((band ? 0 : kDistMax) << (4 * h));
-        }
-        A[i] = out;
+// min of rd(c) and, over 0 < |t| < kDistMax with pos + t in [0, len), of
+// max(|t|, rd(c + t * step)); rd(i) reads cell i of the line's array
+template <class R>
+__device__ __forceinline__ int dist_line(R rd, int c, int pos, int len, int step) {
+    int d = rd(c);
+    for (int t = 1; t < kDistMax && t < d; ++t) {
+        if (pos - t >= 0) d = min(d, max(t, rd(c - t * step)));
+        if (pos + t < len) d = min(d, max(t, rd(c + t * step)));
     }
-    __syncthreads();
-    dist_pass(A, B, n, nx, ny, 0);
-    __syncthreads();
-    dist_pass(B, A, n, nx, ny, 1);
-    __syncthreads();
-    dist_pass(A, B, n, nx, ny, 2);
-    __syncthreads();
-    // whole words (the buffer holds (n + 7) / 8 of them; cells past n read 0)
-    uint32_t *dst = (uint32_t *)P.sdist;
-    for (int i = threadIdx.x; i < (n + 7) / 8; i += blockDim.x) {
-        uint32_t w = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) w |= (4 * i + k < (n + 1) / 2 ? (uint32_t)B[4 * i + k] : 0u) << (8 * k);
-        stg(dst, (size_t)i, w);
-    }
-    if (threadIdx.x == 0) stg(&P.st->sdist_dirty, 0, 0);
+    return d;
 }
 
-void launch_sdist(const SessParams *P, int nsess, int force, hipStream_t s) {
-    if (nsess > 0) hipLaunchKernelGGL(k_sdist, dim3(nsess), dim3(1024), 0, s, P, nsess, force);
+__global__ __launch_bounds__(kSdistThreads) void k_sdist_xy(const SessParams *__restrict__ SP, int force) {
+    __shared__ uint8_t A[kDistCells], B[kDistCells];
+    const SessParams &P = SP[blockIdx.x];
+    const int z = blockIdx.y;
+    if (z >= P.sdim[2] || !sdist_todo(P, force)) return;
+    const int nx = P.sdim[0], ny = P.sdim[1], m = nx * ny, base = z * m;  // m <= kDistCells
+    for (int c = threadIdx.x; c < m; c += kSdistThreads) {
+        const int g = base + c;
+        A[c] = ((ldg(P.sband, g >> 5) >> (g & 31)) & 1u) ? 0 : kDistMax;
+    }
+    __syncthreads();
+    const auto ra = [&](int i) { return (int)A[i]; };
+    for (int c = threadIdx.x; c < m; c += kSdistThreads) B[c] = (uint8_t)dist_line(ra, c, c % nx, nx, 1);
+    __syncthreads();
+    const auto rb = [&](int i) { return (int)B[i]; };
+    for (int c = threadIdx.x; c < m; c += kSdistThreads)
+        stg(P.sdtmp, (size_t)(base + c), (uint8_t)dist_line(rb, c, c / nx, ny, nx));
+}
+
+__global__ __launch_bounds__(kSdistThreads) void k_sdist_z(const SessParams *__restrict__ SP, int force) {
+    const SessParams &P = SP[blockIdx.x];
+    if (!sdist_todo(P, force)) return;
+    const int m = P.sdim[0] * P.sdim[1], nz = P.sdim[2], n = m * nz;
+    const int c0 = blockIdx.y * kSdistChunk + 2 * threadIdx.x;
+    if (c0 >= n) return;
+    const uint8_t *tmp = P.sdtmp;
+    const auto rt = [&](int i) { return (int)ldg(tmp, (size_t)i); };
+    uint8_t out = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int c = c0 + h;
+        if (c < n) out |= (uint8_t)(dist_line(rt, c, c / m, nz, m) << (4 * h));
+    }
+    stg(P.sdist, (size_t)(c0 >> 1), out);
+}
+
+__global__ void k_sdist_done(const SessParams *__restrict__ SP, int nsess) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsess) stg(&SP[s].st->sdist_dirty, 0, 0);
+}
+
+void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipStream_t s) {
+    if (nsess <= 0 || sd_slices <= 0) return;
+    hipLaunchKernelGGL(k_sdist_xy, dim3(nsess, sd_slices), dim3(kSdistThreads), 0, s, P, force);
+    hipLaunchKernelGGL(k_sdist_z, dim3(nsess, kDistCells / kSdistChunk), dim3(kSdistThreads), 0, s, P, force);
+    hipLaunchKernelGGL(k_sdist_done, dim3((nsess + 255) / 256), dim3(256), 0, s, P, nsess);
 }
 
 // ---------------------------------------------------------------------------
@@ -457,7 +464,8 @@ struct Vol {
                 size = kBrick;
             } else {  // the (2d-1)^3 superbricks centred here hold no band brick
                 const int r = d - 1;
-                lo = make_int3(((x >> 5) - r) * kSuper, ((y >> 5) - r) * kSuper, ((z >> 5) - r) * kSuper);
+                constexpr int sh = 3 + kSuperShift;  // voxel -> superbrick
+                lo = make_int3(((x >> sh) - r) * kSuper, ((y >> sh) - r) * kSuper, ((z >> sh) - r) * kSuper);
                 size = (2 * r + 1) * kSuper;
             }
             return 0;
@@ -1080,9 +1088,10 @@ template <class TV, class TW>
 static void integrate_t(const SessParams *P, BrickList bl, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_integrate<TV, TW>), dim3(grid), dim3(kTileThreads), 0, s, P, bl);
 }
-void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tt, int wt, int grid, hipStream_t s) {
+void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tt, int wt, int grid, int sd_slices,
+                      hipStream_t s) {
     VS_DISPATCH(tt, wt, integrate_t, P, bl, grid, s);
-    launch_sdist(P, nsess, 0, s);
+    launch_sdist(P, nsess, sd_slices, 0, s);
 }
 
 template <class TV, class TW>
@@ -1161,9 +1170,10 @@ template <class TV, class TW>
 static void init_t(const SessParams *P, size_t nvox, uint32_t nwords, hipStream_t s) {
     hipLaunchKernelGGL((k_volume_init<TV, TW>), dim3(grid_for(nvox)), dim3(256), 0, s, P, nvox, nwords);
 }
-void launch_volume_init(const SessParams *P, size_t nvox, int tt, int wt, uint32_t nwords, hipStream_t s) {
+void launch_volume_init(const SessParams *P, size_t nvox, int tt, int wt, uint32_t nwords, int sd_slices,
+                        hipStream_t s) {
     VS_DISPATCH(tt, wt, init_t, P, nvox, nwords, s);
-    launch_sdist(P, 1, 1, s);
+    launch_sdist(P, 1, sd_slices, 1, s);
 }
 template <class TV, class TW>
 static void to_lin_t(const SessParams *P, size_t nvox, float *F, float *Wt, hipStream_t s) {
@@ -1184,12 +1194,12 @@ template <class TV, class TW>
 static void occupancy_t(const SessParams *P, uint32_t nbricks, int g, hipStream_t s) {
     hipLaunchKernelGGL((k_occupancy<TV, TW>), dim3(g), dim3(kTileThreads), 0, s, P, nbricks);
 }
-void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tt, int wt, hipStream_t s) {
+void launch_occupancy_rebuild(const SessParams *P, uint32_t nbricks, int tt, int wt, int sd_slices, hipStream_t s) {
     int g = (int)(nbricks > 16384 ? 16384 : nbricks);
     if (g <= 0) return;
     hipLaunchKernelGGL(k_sband_clear, dim3(16), dim3(256), 0, s, P);
     VS_DISPATCH(tt, wt, occupancy_t, P, nbricks, g, s);
-    launch_sdist(P, 1, 1, s);
+    launch_sdist(P, 1, sd_slices, 1, s);
 }
 
 template <class TV, class TW>
