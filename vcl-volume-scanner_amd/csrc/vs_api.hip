@@ -102,8 +102,8 @@ Layout make_layout(const vs_session_desc &d) {
         const size_t n = (size_t)Wl * Hl;
         L.tiles[l] = l < d.levels ? tiles_of(Wl, Hl) : 0;
         L.filt[l] = take(n * 4);
-        L.mvtx[l] = take(n * 16);
-        L.mnrm[l] = take(n * 16);
+        L.mvtx[l] = take(n * sizeof(MapXYZ));
+        L.mnrm[l] = take(n * sizeof(MapXYZ));
         L.mimg[l] = take(n);
     }
     L.tsdf = take(L.nvox * sv);
@@ -879,8 +879,8 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.raw = (const uint16_t *)(m + L.raw);
     for (int l = 0; l < kMaxLevels; ++l) {
         P.filt[l] = (float *)(m + L.filt[l]);
-        P.mvtx[l] = (float4 *)(m + L.mvtx[l]);
-        P.mnrm[l] = (float4 *)(m + L.mnrm[l]);
+        P.mvtx[l] = (MapXYZ *)(m + L.mvtx[l]);
+        P.mnrm[l] = (MapXYZ *)(m + L.mnrm[l]);
         P.mimg[l] = (uint8_t *)(m + L.mimg[l]);
     }
     P.tsdf = m + L.tsdf;
@@ -926,9 +926,9 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
         HIP_TRY(hipMemset(m + L.filt[l], 0, n * 4));
         HIP_TRY(hipMemset(m + L.mimg[l], 0, n));
         if (P.mtex[l]) HIP_TRY(hipMemset(P.mtex[l], 0, n * 3));
-        std::vector<float4> inval(n, make_float4(-100.f, -100.f, -100.f, 0.f));
-        HIP_TRY(hipMemcpy(m + L.mvtx[l], inval.data(), n * 16, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(m + L.mnrm[l], inval.data(), n * 16, hipMemcpyHostToDevice));
+        std::vector<MapXYZ> inval(n, MapXYZ{-100.f, -100.f, -100.f});
+        HIP_TRY(hipMemcpy(m + L.mvtx[l], inval.data(), n * sizeof(MapXYZ), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(m + L.mnrm[l], inval.data(), n * sizeof(MapXYZ), hipMemcpyHostToDevice));
     }
     vs_session *raw = s.release();
     ctx->sessions.push_back(raw);
@@ -1248,22 +1248,13 @@ vs_status vs_get_maps(vs_session *s, int level, float *depth, float *vtx, float 
     vs_status r = sync_last(s);
     if (r != VS_OK) return r;
     const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
-    std::vector<float4> v(n), m(n);
-    HIP_TRY(hipMemcpy(v.data(), s->hp.mvtx[level], n * 16, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(m.data(), s->hp.mnrm[level], n * 16, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n; ++i) {
-        if (depth) depth[i] = (v[i].x == -100.0f) ? 0.0f : v[i].z;
-        if (vtx) {
-            vtx[3 * i] = v[i].x;
-            vtx[3 * i + 1] = v[i].y;
-            vtx[3 * i + 2] = v[i].z;
-        }
-        if (nrm) {
-            nrm[3 * i] = m[i].x;
-            nrm[3 * i + 1] = m[i].y;
-            nrm[3 * i + 2] = m[i].z;
-        }
-    }
+    // the device maps are the ABI's layout (x, y, z per pixel)
+    std::vector<MapXYZ> v(n);
+    HIP_TRY(hipMemcpy(v.data(), s->hp.mvtx[level], n * sizeof(MapXYZ), hipMemcpyDeviceToHost));
+    if (vtx) std::memcpy(vtx, v.data(), n * sizeof(MapXYZ));
+    if (nrm) HIP_TRY(hipMemcpy(nrm, s->hp.mnrm[level], n * sizeof(MapXYZ), hipMemcpyDeviceToHost));
+    if (depth)
+        for (size_t i = 0; i < n; ++i) depth[i] = (v[i].x == -100.0f) ? 0.0f : v[i].z;
     if (img) HIP_TRY(hipMemcpy(img, s->hp.mimg[level], n, hipMemcpyDeviceToHost));
     return VS_OK;
 }
@@ -1274,13 +1265,8 @@ vs_status vs_set_maps(vs_session *s, int level, const float *vtx, const float *n
     vs_status r = sync_last(s);
     if (r != VS_OK) return r;
     const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
-    std::vector<float4> v(n), m(n);
-    for (size_t i = 0; i < n; ++i) {
-        v[i] = make_float4(vtx[3 * i], vtx[3 * i + 1], vtx[3 * i + 2], 0.0f);
-        m[i] = make_float4(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.0f);
-    }
-    HIP_TRY(hipMemcpy(s->hp.mvtx[level], v.data(), n * 16, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->hp.mnrm[level], m.data(), n * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->hp.mvtx[level], vtx, n * sizeof(MapXYZ), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->hp.mnrm[level], nrm, n * sizeof(MapXYZ), hipMemcpyHostToDevice));
     return VS_OK;
 }
 

@@ -75,6 +75,12 @@ struct alignas(16) SessState {
     double last_valid;
 };
 
+// One element of a model vertex or normal map: x, y, z (12 bytes, AoS like
+// the reference's float3 maps); -100 in x marks an invalid pixel.
+struct MapXYZ {
+    float x, y, z;
+};
+
 // Immutable per-session parameters plus buffer pointers; one per session in
 // a device array, so a launch can serve a batch of sessions.
 struct alignas(16) SessParams {
@@ -99,8 +105,8 @@ struct alignas(16) SessParams {
     // buffers (device)
     const uint16_t *raw;     // W*H depth (mm), level-0 size
     float *filt[kMaxLevels];
-    float4 *mvtx[kMaxLevels];  // model vertex (xyz), -100 invalid
-    float4 *mnrm[kMaxLevels];  // model normal (xyz), -100 invalid
+    MapXYZ *mvtx[kMaxLevels];  // model vertex, -100 invalid
+    MapXYZ *mnrm[kMaxLevels];  // model normal, -100 invalid
     uint8_t *mimg[kMaxLevels];
     void *tsdf;              // brick-major values
     void *weight;            // brick-major weights
@@ -198,6 +204,17 @@ __device__ __forceinline__ float4 ldg4(const float4 *p, size_t i) {
 }
 __device__ __forceinline__ void stg4(float4 *p, size_t i, float x, float y, float z, float w) {
     ((VS_GLOBAL vs_f4 *)p)[i] = vs_f4{x, y, z, w};
+}
+// (field by field: one 12-byte access after the compiler merges them)
+__device__ __forceinline__ MapXYZ ldg3(const MapXYZ *p, size_t i) {
+    const VS_GLOBAL MapXYZ *q = (const VS_GLOBAL MapXYZ *)p + i;
+    return MapXYZ{q->x, q->y, q->z};
+}
+__device__ __forceinline__ void stg3(MapXYZ *p, size_t i, float x, float y, float z) {
+    VS_GLOBAL MapXYZ *q = (VS_GLOBAL MapXYZ *)p + i;
+    q->x = x;
+    q->y = y;
+    q->z = z;
 }
 
 // The top 3 rows of a row-major 4x4 FP32 matrix in HBM (16-byte aligned),

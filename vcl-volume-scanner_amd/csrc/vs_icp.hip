@@ -130,10 +130,10 @@ __device__ void publish_T21(SessState *st) {
 // always inside the map (an unused tap re-reads the anchor), so the taps of
 // both maps are loaded together, before any test.
 struct Quad {
-    float4 a, b, c, d;
+    MapXYZ a, b, c, d;
 };
-__device__ __forceinline__ Quad load_quad(const float4 *m, size_t i0, size_t di, size_t dj) {
-    return {ldg4(m, i0), ldg4(m, i0 + di), ldg4(m, i0 + dj), ldg4(m, i0 + di + dj)};
+__device__ __forceinline__ Quad load_quad(const MapXYZ *m, size_t i0, size_t di, size_t dj) {
+    return {ldg3(m, i0), ldg3(m, i0 + di), ldg3(m, i0 + dj), ldg3(m, i0 + di + dj)};
 }
 __device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &out) {
     f3 t1{0, 0, 0}, t2{0, 0, 0}, t3{0, 0, 0}, t4{0, 0, 0};

@@ -832,10 +832,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const bool ok = cast_ray<TV, TW, STATS>(V, C, Tc.m, (float)x, (float)y, p, n_near, n_tri);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
-        stg4(P.mvtx[level], t, p.x, p.y, p.z, 1.0f);
+        stg3(P.mvtx[level], t, p.x, p.y, p.z);  // world hit; the shading makes it the camera-frame vertex
     } else {
-        stg4(P.mvtx[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
-        stg4(P.mnrm[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
+        stg3(P.mvtx[level], t, -100.0f, -100.0f, -100.0f);
+        stg3(P.mnrm[level], t, -100.0f, -100.0f, -100.0f);
         stg(P.mimg[level], t, (uint8_t)0);
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
@@ -860,8 +860,10 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
     const int y = (wi.tile / L.tiles_x) * kTileH + py;
     if (x >= L.W || y >= L.H) return;
     const size_t t = (size_t)y * L.W + x;
-    const float4 h = ldg4(P.mvtx[level], t);
-    if (h.w != 1.0f) return;  // a miss: final already
+    // the march's output (it always runs just before): a world hit point,
+    // or -100 for a miss, which is final already
+    const MapXYZ h = ldg3(P.mvtx[level], t);
+    if (h.x == -100.0f) return;
     const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, nullptr};
     const f3 p = {h.x, h.y, h.z};
     f3 n;
@@ -871,8 +873,8 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
         const f3 pc = transform(T, p);
         const f3 nc = {T[0] * n.x + T[1] * n.y + T[2] * n.z, T[4] * n.x + T[5] * n.y + T[6] * n.z,
                        T[8] * n.x + T[9] * n.y + T[10] * n.z};
-        stg4(P.mvtx[level], t, pc.x, pc.y, pc.z, 0.0f);
-        stg4(P.mnrm[level], t, nc.x, nc.y, nc.z, 0.0f);
+        stg3(P.mvtx[level], t, pc.x, pc.y, pc.z);
+        stg3(P.mnrm[level], t, nc.x, nc.y, nc.z);
         // light (0,0,-1): n.l = 0*nx + 0*ny - nz (z_yooji_object_scanner.cpp:1313)
         const float sa = nc.x * 0.0f + nc.y * 0.0f + nc.z * -1.0f;
         stg(P.mimg[level], t, (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f))));
@@ -883,8 +885,8 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
         }
         if (STATS) atomicAdd(stats + 2, 1ull);
     } else {
-        stg4(P.mvtx[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
-        stg4(P.mnrm[level], t, -100.0f, -100.0f, -100.0f, 0.0f);
+        stg3(P.mvtx[level], t, -100.0f, -100.0f, -100.0f);
+        stg3(P.mnrm[level], t, -100.0f, -100.0f, -100.0f);
         stg(P.mimg[level], t, (uint8_t)0);
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
