@@ -135,22 +135,16 @@ struct Quad {
 __device__ __forceinline__ Quad load_quad(const MapXYZ *m, size_t i0, size_t di, size_t dj) {
     return {ldg3(m, i0), ldg3(m, i0 + di), ldg3(m, i0 + dj), ldg3(m, i0 + di + dj)};
 }
+// Taps are picked by selects, not branches, so that no tap's load is sunk
+// into a branch of its own (and waited for alone).
 __device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &out) {
-    f3 t1{0, 0, 0}, t2{0, 0, 0}, t3{0, 0, 0}, t4{0, 0, 0};
-    bool ok = q.a.x != -100.0f;
-    t1 = {q.a.x, q.a.y, q.a.z};
-    if (rx > 0.0f) {
-        ok &= q.b.x != -100.0f;
-        t2 = {q.b.x, q.b.y, q.b.z};
-    }
-    if (ry > 0.0f) {
-        ok &= q.c.x != -100.0f;
-        t3 = {q.c.x, q.c.y, q.c.z};
-    }
-    if (rx > 0.0f && ry > 0.0f) {
-        ok &= q.d.x != -100.0f;
-        t4 = {q.d.x, q.d.y, q.d.z};
-    }
+    const bool ux = rx > 0.0f, uy = ry > 0.0f, uxy = ux & uy;
+    const bool ok = (q.a.x != -100.0f) & (!ux | (q.b.x != -100.0f)) & (!uy | (q.c.x != -100.0f)) &
+                    (!uxy | (q.d.x != -100.0f));
+    const f3 t1 = {q.a.x, q.a.y, q.a.z};
+    const f3 t2 = {ux ? q.b.x : 0.0f, ux ? q.b.y : 0.0f, ux ? q.b.z : 0.0f};
+    const f3 t3 = {uy ? q.c.x : 0.0f, uy ? q.c.y : 0.0f, uy ? q.c.z : 0.0f};
+    const f3 t4 = {uxy ? q.d.x : 0.0f, uxy ? q.d.y : 0.0f, uxy ? q.d.z : 0.0f};
     const float ix = 1.0f - rx, iy = 1.0f - ry;
     out = add(scale(iy, add(scale(ix, t1), scale(rx, t2))), scale(ry, add(scale(ix, t3), scale(rx, t4))));
     return ok;
@@ -161,11 +155,11 @@ __device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &ou
 // folded into `ok`; the model taps use coordinates clamped into the map, so
 // every load is in range whatever the tests say, and a rejected pixel's
 // Jacobian and residual are zero.
-__device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, int y, const float *T21, float A[6],
-                                            float &b) {
+// td1: the filtered depth at (x, y), loaded by the caller
+__device__ __forceinline__ bool icp_summand_d(const SessParams &P, int l, int x, int y, float td1, const float *T21,
+                                              float A[6], float &b) {
     const LevelDesc &L = P.lev[l];
     const int W = L.W, H = L.H;
-    const float td1 = ldg(P.filt[l], (size_t)y * W + x);
     const f3 p = transform(T21, back_project((float)x, (float)y, L.K, td1));
     float u, v;
     project(p, L.K, u, v);
@@ -191,6 +185,10 @@ __device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, i
     A[5] = ok ? -n.z : 0.0f;
     b = ok ? n.x * dx + n.y * dy + n.z * dz : 0.0f;
     return ok;
+}
+__device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, int y, const float *T21, float A[6],
+                                            float &b) {
+    return icp_summand_d(P, l, x, y, ldg(P.filt[l], (size_t)y * P.lev[l].W + x), T21, A, b);
 }
 
 // Adds one pixel's 29 terms (JtJ upper triangle, Jtr, count, r^2) to the
@@ -271,13 +269,24 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     float acc[kIcpVals];
 #pragma unroll
     for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0f;
-    for (int g = 0; g < kIcpGroup && t0 + g < ntiles; ++g) {
+    // the group's depths first (independent loads; a pixel outside the
+    // frame re-reads pixel 0 and is skipped), then the pixels in order
+    int xs[kIcpGroup], ys[kIcpGroup];
+    bool in[kIcpGroup];
+    float td[kIcpGroup];
+#pragma unroll
+    for (int g = 0; g < kIcpGroup; ++g) {
         const int tile = t0 + g;
-        const int x = (tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
-        const int y = (tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
-        if (x < L.W && y < L.H) {
+        xs[g] = (tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
+        ys[g] = (tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
+        in[g] = tile < ntiles && xs[g] < L.W && ys[g] < L.H;
+        td[g] = ldg(P.filt[level], in[g] ? (size_t)ys[g] * L.W + xs[g] : (size_t)0);
+    }
+#pragma unroll
+    for (int g = 0; g < kIcpGroup; ++g) {
+        if (in[g]) {
             float A[6], b;
-            const bool ok = icp_summand(P, level, x, y, T21s, A, b);
+            const bool ok = icp_summand_d(P, level, xs[g], ys[g], td[g], T21s, A, b);
             add_terms(ok, A, b, acc);
         }
     }
