@@ -235,23 +235,17 @@ def _track_parity(sess, stream):
     assert gt_t < 0.01 and gt_a < 1.0, (gt_t, gt_a)
 
 
-def test_noncubic_grid_odd_frame(ctx):
-    """A grid of 120 x 104 x 72 voxels (sides not multiples of the 16-voxel
-    superbrick, so the last superbrick of each axis is partly outside) and a
-    200x150 frame (levels 100x75 and 50x37, odd halvings): integrate's brick
-    mask, visible and updated counts and weights bit-exact and the TSDF within
-    1e-4 of the oracle; on the oracle's volume, the raycast's hit masks
-    bit-exact and points within 0.5 voxel at every level. Cubic grids and
-    even frames cannot catch an axis mix-up in the brick or superbrick
-    indexing."""
-    print("case: non-cubic 120x104x72 grid, 200x150 frame", flush=True)
-    st = vscan.SceneStream(0, seed=2, width=200, height=150)
-    d = vscan.make_desc(W0, K=st.K, W=200, H=150)
-    d.vol_dim[0], d.vol_dim[1], d.vol_dim[2] = 120, 104, 72
+def _grid_parity(ctx, st, d, shapes):
+    """Integrate frames 0, 4, 8 at ground-truth poses and compare with the
+    oracle (brick mask, visible and updated counts and weights bit-exact,
+    TSDF within 1e-4), then raycast the oracle's volume at pose 10 (hit masks
+    bit-exact, points within 0.5 voxel) at every level; `shapes` are the
+    expected (width, height) of the levels."""
+    nx, ny, nz = d.vol_dim[0], d.vol_dim[1], d.vol_dim[2]
     s = vscan.Session(ctx, d, T_gc0=st.pose(0))
     try:
         vol = orc.volume(d)
-        N = 120 * 104 * 72
+        N = nx * ny * nz
         assert s.nvox == N
         F = np.ones(N)
         Wt = np.zeros(N)
@@ -265,7 +259,7 @@ def test_noncubic_grid_odd_frame(ctx):
             Tg, Tc = f32pose(T)
             bm, upd, counts = orc.integrate(mm, K[0], Tg, Tc, vol, F, Wt)
             gF, gW, gbm = s.volume()
-            assert gF.shape == (72, 104, 120)
+            assert gF.shape == (nz, ny, nx)
             assert np.array_equal(gbm, bm), f"brick mask differs at frame {fr}"
             assert s.integrate_counts() == (counts[0], counts[1]), (s.integrate_counts(), counts)
             assert np.array_equal(gW.reshape(-1), Wt.astype(np.float32)), "weights differ"
@@ -280,7 +274,7 @@ def test_noncubic_grid_odd_frame(ctx):
             s.raycast(l)
             _, gv, _, _ = s.maps(l)
             h, w = s.level_shape(l)
-            assert (w, h) == ((200, 150), (100, 75), (50, 37))[l]
+            assert (w, h) == shapes[l]
             _, rv, _, _, _ = orc.raycast(F32, W32, vol, w, h, K[l], Tg, Tc)
             gm, rm = gv[..., 0] != -100, rv[..., 0] != -100
             assert np.array_equal(gm, rm), f"level {l}: {int(np.sum(gm != rm))} rays differ in hit/miss"
@@ -288,3 +282,29 @@ def test_noncubic_grid_odd_frame(ctx):
             assert np.abs(gv - rv)[rm].max() <= 0.5 * d.voxel
     finally:
         s.close()
+
+
+def test_noncubic_grid_odd_frame(ctx):
+    """A grid of 120 x 104 x 72 voxels (sides not multiples of the 16-voxel
+    superbrick, so the last superbrick of each axis is partly outside) and a
+    200x150 frame (levels 100x75 and 50x37, odd halvings). Cubic grids and
+    even frames cannot catch an axis mix-up in the brick or superbrick
+    indexing."""
+    print("case: non-cubic 120x104x72 grid, 200x150 frame", flush=True)
+    st = vscan.SceneStream(0, seed=2, width=200, height=150)
+    d = vscan.make_desc(W0, K=st.K, W=200, H=150)
+    d.vol_dim[0], d.vol_dim[1], d.vol_dim[2] = 120, 104, 72
+    _grid_parity(ctx, st, d, ((200, 150), (100, 75), (50, 37)))
+
+
+def test_longest_grid_side(ctx):
+    """A grid at the largest side the ABI accepts, 2048 voxels along x (256
+    bricks, 128 superbricks; 2048 x 64 x 64, a 16 m strip through the scene):
+    the largest brick coordinates and a ray start 7 m behind the camera (the
+    cube's bounding sphere), with the same bars as above."""
+    print("case: 2048x64x64 grid, 320x240 frame", flush=True)
+    st = vscan.SceneStream(0, seed=3)
+    d = vscan.make_desc(W0, K=st.K)
+    d.vol_dim[0], d.vol_dim[1], d.vol_dim[2] = 2048, 64, 64
+    d.origin[0], d.origin[1], d.origin[2] = -1024 * d.voxel, -32 * d.voxel, -0.05
+    _grid_parity(ctx, st, d, ((320, 240), (160, 120), (80, 60)))
