@@ -40,8 +40,8 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
 // Marks the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
 // pixel's ray and appends each newly marked brick once to the batch's list.
 // Neighbouring pixels hit the same few bricks, so a workgroup first
-// deduplicates its up to 6 x 256 samples in an LDS hash set; only the first
-// insertion of a brick touches the device-wide bitmap (device-scope atomics are
+// deduplicates its up to 6 x 256 samples in an LDS hash set; then each brick
+// of the set touches the device-wide bitmap once (device-scope atomics are
 // performed past the per-XCD L2 and are the expensive part). The bricks this
 // workgroup marks first are gathered in LDS and appended with one reservation
 // on the list counter (one same-address atomic per workgroup, not per brick).
@@ -83,20 +83,36 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
             uint32_t h = (b * 2654435761u) >> 21;  // 11-bit multiplicative hash
             while (true) {
                 const int old = atomicCAS(&set[h], -1, (int)b);
-                if (old == -1) {
-                    // first sample of this brick in the workgroup: set its bit
-                    // bits are only set during this launch (cleared by the
-                    // launch before): a plain cached read that shows the bit
-                    // proves it set; a stale 0 costs one atomic
-                    const uint32_t bit = 1u << (b & 31u);
-                    uint32_t *word = P.bvis + (b >> 5);
-                    if (!(ldg(P.bvis, b >> 5) & bit) && !(atomicOr(word, bit) & bit))
-                        fresh[atomicAdd(&nfresh, 1u)] = (int)b;
-                    break;
-                }
-                if (old == (int)b) break;
+                if (old == -1 || old == (int)b) break;
                 h = (h + 1) & (kVisSet - 1);
             }
+        }
+    }
+    __syncthreads();
+    // Set the device bits of the workgroup's bricks: each thread takes
+    // kVisSet / kTileThreads slots and issues their hint reads, then their
+    // returning atomics, together (device-scope atomics are slow; waiting
+    // for each before issuing the next made them a serial chain). Bits are
+    // only set during this launch (cleared by the launch before): a plain
+    // cached read that shows the bit proves it set; a stale 0 costs one atomic.
+    {
+        constexpr int kPer = kVisSet / kTileThreads;
+        int bs[kPer];
+        uint32_t hint[kPer], got[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            bs[j] = set[threadIdx.x + j * kTileThreads];
+            hint[j] = bs[j] >= 0 ? ldg(P.bvis, (uint32_t)bs[j] >> 5) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t bit = bs[j] >= 0 ? 1u << ((uint32_t)bs[j] & 31u) : 1u;
+            got[j] = (hint[j] & bit) ? bit : atomicOr(P.bvis + ((uint32_t)bs[j] >> 5), bit);
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t bit = bs[j] >= 0 ? 1u << ((uint32_t)bs[j] & 31u) : 1u;
+            if (bs[j] >= 0 && !(got[j] & bit)) fresh[atomicAdd(&nfresh, 1u)] = bs[j];
         }
     }
     __syncthreads();
