@@ -247,7 +247,7 @@ typedef enum { VS_TUNE_OFF = 0, VS_TUNE_ONLINE = 1, VS_TUNE_CACHED = 2 } vs_tune
 vs_status vs_ctx_set_tuning(vs_ctx *ctx, int mode, const char *cache_path);
 /* The context's settled shapes, one size class per line:
  * "<class>|<march block>|<shade block>|<integrate workgroups>|<means>", where
- * <means> is "m8x8,m16x4,m32x2|s32x2,s8x8|i1024,i2048,i4096,i8192" (mean stage
+ * <means> is "m8x8,m16x4,m32x2|s32x2,s8x8|i1024,i2048,i4096,i8192,i16384,i32768" (mean stage
  * ms per candidate, -1 for one not timed) or "cached" (from the file). Writes
  * at most len bytes including the terminating zero; *needed (nullable) = the
  * full length + 1. */

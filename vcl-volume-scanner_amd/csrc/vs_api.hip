@@ -199,8 +199,8 @@ struct Shape {
 };
 constexpr int kMarchOpts[] = {kRay8x8, kRay16x4, kRay32x2};
 constexpr int kShadeOpts[] = {kRay32x2, kRay8x8};
-constexpr int kIntegGrids[] = {1024, 2048, 4096, 8192};
-constexpr int kNumMarch = 3, kNumShade = 2, kNumInteg = 4;
+constexpr int kIntegGrids[] = {1024, 2048, 4096, 8192, 16384, 32768};
+constexpr int kNumMarch = 3, kNumShade = 2, kNumInteg = 6;  // kTuneSteps is a multiple of each
 const char *const kRayBlockName[] = {"8x8", "16x4", "32x2"};
 // Online tuning: the first kTuneSteps steps of a batch whose size class has
 // no choice yet run eagerly and timed, step i with march candidate i % 3,
@@ -208,10 +208,13 @@ const char *const kRayBlockName[] = {"8x8", "16x4", "32x2"};
 // stage is timed 2 to 6 times, on consecutive frames); then each stage keeps
 // the candidate with the lowest mean time and the step is captured with it.
 constexpr int kTuneSteps = 12;
+static_assert(kTuneSteps % kNumMarch == 0 && kTuneSteps % kNumShade == 0 && kTuneSteps % kNumInteg == 0,
+              "every candidate gets the same number of tuning steps");
 struct TuneState {
     int step = 0;           // tuning steps enqueued
-    double sum[3][4] = {};  // [march, shade, integrate][candidate] ms
-    int cnt[3][4] = {};
+    static constexpr int kCand = std::max(kNumMarch, std::max(kNumShade, kNumInteg));
+    double sum[3][kCand] = {};  // [march, shade, integrate][candidate] ms
+    int cnt[3][kCand] = {};
     int ci[3] = {0, 0, 0};  // candidates of the step in flight
     bool pending = false;   // that step's events are not read yet
     Timer timer;

@@ -302,6 +302,9 @@ class Context:
     def set_timing(self, on):
         check(lib().vs_ctx_set_timing(self.h, 1 if on else 0), "vs_ctx_set_timing")
 
+    # integrate workgroup counts the tuner tries, in the order of vs_ctx_tuning's means
+    INTEG_GRIDS = (1024, 2048, 4096, 8192, 16384, 32768)
+
     def set_tuning(self, mode=VS_TUNE_ONLINE, cache_path=None):
         """Launch-shape autotuning (vs_ctx_set_tuning)."""
         path = cache_path.encode() if cache_path else None
@@ -322,7 +325,7 @@ class Context:
             if f[4] != "cached":
                 m, sh, ig = ([float(x) for x in g.split(",")] for g in f[4:7])
                 ms = {"march": dict(zip(["8x8", "16x4", "32x2"], m)), "shade": dict(zip(["32x2", "8x8"], sh)),
-                      "integrate": dict(zip([1024, 2048, 4096, 8192], ig))}
+                      "integrate": dict(zip(self.INTEG_GRIDS, ig))}
             out[f[0]] = {"march": f[1], "shade": f[2], "integrate": int(f[3]), "ms": ms}
         return out
 
