@@ -6,7 +6,7 @@
 // block partials to the host, sums them in FP32, solves on the host and copies
 // the pose back. Here each iteration is two launches: k_icp_iter writes one
 // FP64 partial of the 29 sums per group of tiles, then k_icp_solve (one
-// workgroup per session) sums them in FP64 in group order, solves the 6x6
+// workgroup per session) sums them in FP64 in a fixed order, solves the 6x6
 // system (Cholesky, FP64), applies the SE(3) update and publishes the FP32
 // pose the next iteration reads. Convergence (|x|_inf < 5e-7) and failure are
 // flags the following launches test, so the host enqueues a fixed launch
@@ -295,7 +295,8 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
 }
 
 // Second half: one workgroup per session sums the group partials in FP64 in
-// group order (so the result does not depend on the batch or the schedule),
+// a fixed order (thread t adds groups t, t + 256, ..., then a fixed tree over
+// the threads; the result does not depend on the batch or the schedule),
 // solves the 6x6 system and applies the SE(3) update.
 __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__restrict__ SP, int nsess, int level,
                                                             int iter) {
