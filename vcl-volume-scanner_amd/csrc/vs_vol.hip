@@ -527,7 +527,8 @@ struct RayConst {
 // estimates (relative 2^-22 of at most 64 steps), so each skipped step's tap
 // would have read 1.0 too. The skipped additions are performed one by
 // one (a closed form per binade costs more VALU than the
-// handful of additions a box skips on average).
+// handful of additions a box skips on average), four per loop trip: the
+// loop's own counter, compare and branch cost a wave as much as the adds.
 constexpr int kSkipMax = 64;
 constexpr float kSkipMargin = 0.25f;
 
@@ -632,7 +633,21 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
             if (k > 0) {
                 iter += k;
                 if (iter > C.max_iter) return false;
-                for (int j = 0; j < k; ++j) {
+                int j = k;
+                for (; j >= 4; j -= 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        vox = add(vox, d);
+                        total += S;
+                    }
+                }
+                if (j & 2) {
+                    vox = add(vox, d);
+                    total += S;
+                    vox = add(vox, d);
+                    total += S;
+                }
+                if (j & 1) {
                     vox = add(vox, d);
                     total += S;
                 }
