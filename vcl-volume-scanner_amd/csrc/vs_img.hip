@@ -12,6 +12,29 @@ namespace vs {
 
 constexpr int kHalo = 3;  // largest bilateral radius supported
 
+// The 2R+1 x 2R+1 window of one pixel (tile coordinates of the centre): sum
+// of w and of w (d - c), and whether a tap is <= 0. RR > 0: the radius at
+// compile time (the loops unroll); RR = 0: R at run time. Same operations in
+// the same order either way.
+template <int RR, int TW>
+__device__ __forceinline__ void bil_window(const float (*tile)[TW], int cy, int cx, int R, const float *g, float c,
+                                           float inv2s2, float &s, float &t, bool &zero) {
+    const int r = RR > 0 ? RR : R;
+#pragma unroll
+    for (int i = -r; i <= r; ++i) {
+        const float gi = g[i + r];
+#pragma unroll
+        for (int j = -r; j <= r; ++j) {
+            const float p = tile[cy + i][cx + j];
+            zero |= (p <= 0.0f);
+            const float d = p - c;
+            const float w = gi * g[j + r] * __expf(-(d * d) * inv2s2);
+            t = fmaf(w, d, t);
+            s += w;
+        }
+    }
+}
+
 // Bilateral filter of the raw uint16 mm frame into pyramid level 0 (metres).
 // Reference: d_bilateral_filter (z_yooji_cuda_img_proc.cu:103-141): weight
 // g[i] g[j] exp(-(d - c)^2 / (2 sigma_r^2)), output 0 if any tap <= 0. Taps are
@@ -43,17 +66,19 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
     const float c = tile[ly + kHalo][lx + kHalo];
     float s = 0.0f, t = 0.0f;
     bool zero = false;
-    for (int i = -R; i <= R; ++i) {
-        const float gi = P.bil_g[i + R];
-        for (int j = -R; j <= R; ++j) {
-            const float p = tile[ly + kHalo + i][lx + kHalo + j];
-            zero |= (p <= 0.0f);
-            const float d = p - c;
-            const float w = gi * P.bil_g[j + R] * __expf(-(d * d) * P.bil_inv2s2);
-            t = fmaf(w, d, t);
-            s += w;
-        }
-    }
+    float g[2 * kHalo + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * kHalo + 1; ++k) g[k] = P.bil_g[k];
+    const float inv2s2 = P.bil_inv2s2;
+    const int cy = ly + kHalo, cx = lx + kHalo;
+    if (R == 2)
+        bil_window<2, TW>(tile, cy, cx, R, g, c, inv2s2, s, t, zero);
+    else if (R == 1)
+        bil_window<1, TW>(tile, cy, cx, R, g, c, inv2s2, s, t, zero);
+    else if (R == 3)
+        bil_window<3, TW>(tile, cy, cx, R, g, c, inv2s2, s, t, zero);
+    else  // run-time radius: weights read where they live (no dynamic register index)
+        bil_window<0, TW>(tile, cy, cx, R, P.bil_g, c, inv2s2, s, t, zero);
     stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
 }
 
