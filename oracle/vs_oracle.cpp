@@ -43,6 +43,23 @@
 //  Q18 __float2int_rn of a non-finite or |v| >= 2^31 value is "out of range".
 #include "vs_oracle.h"
 
+// OR_MUTATION (0 = none) builds a deliberately wrong oracle for the pin
+// tests' own check: each pin must fail on the mutant that breaks the function
+// it pins (oracle/Makefile target `mutants`, tests/test_oracle_mutants.py).
+//   1  A13: samples n = -1..1 (the CPU twin's range) instead of -4..1
+//   2  A13: ROUNDF(v)/8 with the macro parenthesised (no precedence quirk)
+//   3  A17: one of the 32 gradient taps read one voxel off along x
+//   4  A18: shading factor 0.75 instead of 0.8
+#ifndef OR_MUTATION
+#define OR_MUTATION 0
+#endif
+// OR_MARCH_CAP_MULT (1 = decision Q13) scales the march's iteration cap, to
+// check on the test scenes that the cap decides no ray (a build with a cap
+// 16x longer gives the same maps).
+#ifndef OR_MARCH_CAP_MULT
+#define OR_MARCH_CAP_MULT 1
+#endif
+
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -576,7 +593,7 @@ void mark_bricks(const uint16_t *mm, int W, int H, const float *K, const float *
             f3 p = transform(Tcg, back_project((float)x, (float)y, K, td));
             f3 pe = transform(Tcg, back_project((float)x, (float)y, K, 100.0f));
             f3 dir = normalised(sub(pe, p));
-            for (int n = -4; n <= 1; ++n) {
+            for (int n = (OR_MUTATION == 1 ? -1 : -4); n <= 1; ++n) {
                 f3 q = add(p, scale((float)n * V.mu, dir));
                 f3 tv;
                 if (!world_to_voxel(V, q, tv)) continue;
@@ -584,7 +601,11 @@ void mark_bricks(const uint16_t *mm, int W, int H, const float *K, const float *
                 const float c[3] = {tv.x, tv.y, tv.z};
                 for (int k = 0; k < 3; ++k) {
                     // ROUNDF(v)/8 with the macro's precedence (Q6)
+#if OR_MUTATION == 2
+                    bxyz[k] = ((c[k] < 0.0f) ? (int)(c[k] - 0.5f) : (int)(c[k] + 0.5f)) / 8;
+#else
                     bxyz[k] = (c[k] < 0.0f) ? (int)(c[k] - 0.5f) : (int)(c[k] + 0.5f) / 8;
+#endif
                 }
                 if (bxyz[0] < 0 || bxyz[0] >= V.BW || bxyz[1] < 0 || bxyz[1] >= V.BH || bxyz[2] < 0 ||
                     bxyz[2] >= V.BD)
@@ -823,7 +844,7 @@ RayParams ray_params(const Vol &V, const float *K, const float *Tgc, const float
     P.cos_max = (s < 1.0f) ? sqrtf(1.0f - s * s) : -2.0f;
     P.step_scale = 0.9f * V.mu * V.vox_inv;
     P.total_max = 2.0f * P.r_cube * V.vox_inv;
-    P.max_iter = (int)(P.total_max / std::min(P.step_scale, 1.0f)) + 64;
+    P.max_iter = (int)(P.total_max / std::min(P.step_scale, 1.0f)) * OR_MARCH_CAP_MULT + 64;
     return P;
 }
 
@@ -891,7 +912,11 @@ inline bool surface_normal(const Cast &c, f3 p, R n[3]) {
     f3 r;
     to_int(vox, o, r);
     static const int dx1[4] = {0, 1, 0, 1}, dy1[4] = {0, 0, 1, 1};
+#if OR_MUTATION == 3
+    static const int dx2[12] = {0, 1, -1, 0, 1, 1, -1, 0, 1, 2, 0, 1};
+#else
     static const int dx2[12] = {0, 1, -1, 0, 1, 2, -1, 0, 1, 2, 0, 1};
+#endif
     static const int dy2[12] = {-1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2};
     float fm[4], fr[12], bk[12], bm[4];
     for (int i = 0; i < 4; ++i)
@@ -1083,7 +1108,7 @@ void raycast(const float *F, const float *Wt, const or_volume *vol, int W, int H
                 }
                 // light (0,0,-1) (z_yooji_object_scanner.cpp:1313)
                 R sa = -nc[2];
-                R v = (R(0.8) * sa + R(0.2)) * R(255);
+                R v = (R(OR_MUTATION == 4 ? 0.75 : 0.8) * sa + R(0.2)) * R(255);
                 v = std::max(R(0), std::min(R(255), v));
                 if (img) img[t] = (uint8_t)v;
                 if (tex) {

@@ -4,6 +4,7 @@ own host math compiled into oracle/_ref/libref_math.so.
 Test infrastructure only: the oracle is the checker the HIP pipeline is
 compared against.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -46,41 +47,58 @@ def oracle():
     if _or is None:
         if not os.path.exists(ORACLE_PATH):
             raise RuntimeError("oracle not built: make -C oracle")
-        L = C.CDLL(ORACLE_PATH)
-        L.or_icp_system_d.restype = C.c_longlong
-        L.or_icp_system_d.argtypes = [P, C.c_int, C.c_int, P, P, P, P, C.c_float, P, P, P]
-        L.or_bilateral_d.argtypes = [P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P]
-        L.or_bilateral_f.argtypes = [P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P]
-        L.or_gaussian_weights.argtypes = [C.c_float, C.c_int, P]
-        L.or_pyr_down_d.argtypes = [P, C.c_int, C.c_int, P]
-        L.or_pyr_down_f.argtypes = [P, C.c_int, C.c_int, P]
-        L.or_intrinsics_pyramid.argtypes = [P, C.c_int, P]
-        L.or_vnmap_d.argtypes = [P, C.c_int, C.c_int, P, C.c_int, P, P]
-        L.or_cholesky_solve.argtypes = [P, P, P]
-        L.or_cholesky_solve.restype = C.c_int
-        L.or_expmap.argtypes = [P, P]
-        L.or_rigid_inverse.argtypes = [P, P]
-        L.or_matmul4.argtypes = [P, P, P]
-        L.or_icp_track_d.argtypes = [P, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_float, C.c_int, C.c_float,
-                                     C.c_float, P, P, P]
-        L.or_icp_track_d.restype = C.c_int
-        L.or_integrate_d.argtypes = [P, C.c_int, C.c_int, P, P, P, C.POINTER(OrVolume), P, P, P, P, P]
-        L.or_integrate_f.argtypes = [P, C.c_int, C.c_int, P, P, P, C.POINTER(OrVolume), P, P, P, P, P]
-        L.or_raycast_d.argtypes = [P, P, C.POINTER(OrVolume), C.c_int, C.c_int, P, P, P, P, P, P, P, P]
-        L.or_raycast_f.argtypes = [P, P, C.POINTER(OrVolume), C.c_int, C.c_int, P, P, P, P, P, P, P, P]
-        L.or_session_create.argtypes = [C.POINTER(OrSessionDesc), P]
-        L.or_session_create.restype = P
-        L.or_session_destroy.argtypes = [P]
-        L.or_fuse_frame.argtypes = [P, P, P]
-        L.or_fuse_frame.restype = C.c_int
-        L.or_session_pose.argtypes = [P, P]
-        L.or_session_maps.argtypes = [P, C.c_int, P, P, P]
-        L.or_session_volume.argtypes = [P, P, P]
-        L.or_session_filtered.argtypes = [P, C.c_int, P]
-        L.or_depth_to_m.argtypes = [P, C.c_longlong, P]
-        L.or_num_threads.restype = C.c_int
-        _or = L
+        _or = _load_oracle(ORACLE_PATH)
     return _or
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every oracle call of this module through the build at `path`
+    (one of the variant builds of oracle/Makefile) inside the block."""
+    global _or
+    saved = oracle()
+    _or = _load_oracle(path)
+    try:
+        yield
+    finally:
+        _or = saved
+
+
+def _load_oracle(path):
+    L = C.CDLL(path)
+    L.or_icp_system_d.restype = C.c_longlong
+    L.or_icp_system_d.argtypes = [P, C.c_int, C.c_int, P, P, P, P, C.c_float, P, P, P]
+    L.or_bilateral_d.argtypes = [P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P]
+    L.or_bilateral_f.argtypes = [P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, P]
+    L.or_gaussian_weights.argtypes = [C.c_float, C.c_int, P]
+    L.or_pyr_down_d.argtypes = [P, C.c_int, C.c_int, P]
+    L.or_pyr_down_f.argtypes = [P, C.c_int, C.c_int, P]
+    L.or_intrinsics_pyramid.argtypes = [P, C.c_int, P]
+    L.or_vnmap_d.argtypes = [P, C.c_int, C.c_int, P, C.c_int, P, P]
+    L.or_cholesky_solve.argtypes = [P, P, P]
+    L.or_cholesky_solve.restype = C.c_int
+    L.or_expmap.argtypes = [P, P]
+    L.or_rigid_inverse.argtypes = [P, P]
+    L.or_matmul4.argtypes = [P, P, P]
+    L.or_icp_track_d.argtypes = [P, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_float, C.c_int, C.c_float,
+                                 C.c_float, P, P, P]
+    L.or_icp_track_d.restype = C.c_int
+    L.or_integrate_d.argtypes = [P, C.c_int, C.c_int, P, P, P, C.POINTER(OrVolume), P, P, P, P, P]
+    L.or_integrate_f.argtypes = [P, C.c_int, C.c_int, P, P, P, C.POINTER(OrVolume), P, P, P, P, P]
+    L.or_raycast_d.argtypes = [P, P, C.POINTER(OrVolume), C.c_int, C.c_int, P, P, P, P, P, P, P, P]
+    L.or_raycast_f.argtypes = [P, P, C.POINTER(OrVolume), C.c_int, C.c_int, P, P, P, P, P, P, P, P]
+    L.or_session_create.argtypes = [C.POINTER(OrSessionDesc), P]
+    L.or_session_create.restype = P
+    L.or_session_destroy.argtypes = [P]
+    L.or_fuse_frame.argtypes = [P, P, P]
+    L.or_fuse_frame.restype = C.c_int
+    L.or_session_pose.argtypes = [P, P]
+    L.or_session_maps.argtypes = [P, C.c_int, P, P, P]
+    L.or_session_volume.argtypes = [P, P, P]
+    L.or_session_filtered.argtypes = [P, C.c_int, P]
+    L.or_depth_to_m.argtypes = [P, C.c_longlong, P]
+    L.or_num_threads.restype = C.c_int
+    return L
 
 
 _ref = None
