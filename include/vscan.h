@@ -21,7 +21,8 @@
  *
  * Conventions (same as the reference):
  *   - depth input is uint16 millimetres, converted to metres as mm * 0.001f
- *     (STAC/Util/z_yooji_frame_capture.cpp:597,626); 0 = invalid;
+ *     (STAC/Util/z_yooji_frame_capture.cpp:597,626), 0 = invalid, or float
+ *     metres as the reference's operators take them (vs_depth_units);
  *   - T_gc maps world ("global") points into the camera frame, row-major 4x4;
  *     T_cg is its inverse;
  *   - voxel (i,j,k) has its centre at origin + (i+0.5)*voxel; the linear index
@@ -50,7 +51,9 @@ typedef enum {
     VS_TRACK_LOST = 4,      /* Cholesky pivot below 1e-8 (or non-finite): pose kept, frame not fused */
     VS_MOTION_REJECTED = 5, /* optional motion gate fired: pose kept, frame not fused */
     VS_BUDGET = 6,          /* HBM budget exhausted (serving admission) */
-    VS_TOO_FEW_POINTS = 7   /* optional minimum-correspondence gate fired */
+    VS_TOO_FEW_POINTS = 7,  /* optional minimum-correspondence gate fired */
+    VS_SINGULAR = 8         /* optional relative pivot guard fired (icp_rel_pivot): the normal equations
+                               are nearly rank deficient (e.g. one plane in view); pose kept, frame not fused */
 } vs_status;
 
 typedef enum { VS_TSDF_F32 = 0, VS_TSDF_F16 = 1 } vs_tsdf_type;
@@ -64,6 +67,17 @@ typedef enum { VS_WEIGHT_F32 = 0, VS_WEIGHT_U8 = 1 } vs_weight_type;
  *     voxels at the weight cap frozen, an RGB running average per voxel, and a
  *     rendered texture map per pyramid level (tsdf_renderer.cu:1166-1184). */
 typedef enum { VS_INTEGRATE_DEPTH = 0, VS_INTEGRATE_COLOR = 1 } vs_integrate_mode;
+/* Units of the session's depth frames (every entry point that takes a depth
+ * frame of the session's size reads it in these units):
+ *   VS_DEPTH_MM_U16: uint16 millimetres, converted as mm * 0.001f (the capture
+ *     path, z_yooji_frame_capture.cpp:597,626); 0 = invalid;
+ *   VS_DEPTH_M_F32: float metres, as the reference's operators take them
+ *     (bfd_Bilateral_Filter_Depth's host_in, cdtoc's depth_dev); any value that
+ *     is not a finite positive number (0, negative, NaN, infinite) is invalid.
+ * Frames passed through the uint16_t pointers of vs_set_depth, vs_fuse_frame,
+ * vs_fuse_batch and vs_server_step are read in the session's units (cast a
+ * float frame's pointer). */
+typedef enum { VS_DEPTH_MM_U16 = 0, VS_DEPTH_M_F32 = 1 } vs_depth_units;
 
 /* One scan session: camera, pyramid, tracker, volume and filter settings.
  * Replaces CKvYooji_Scanner_Parameter (STAC/Object/CPU/_yooji_scanner_parameter.h:4-71)
@@ -91,6 +105,10 @@ typedef struct {
     float bil_sigma_r;           /* bilateral range sigma (m) */
     int bil_radius;              /* bilateral half window (1..3) */
     int integrate_mode;          /* vs_integrate_mode */
+    int depth_units;             /* vs_depth_units of the session's depth frames */
+    float icp_rel_pivot;         /* relative pivot guard of the ICP solve: VS_SINGULAR when a squared
+                                    Cholesky pivot L_jj^2 < icp_rel_pivot * A_jj; 0 = off (the reference
+                                    tests only the absolute pivot, L_jj < 1e-8) */
 } vs_session_desc;
 
 /* Fill *d with the reference's defaults for a W x H camera and an N^3 cube of
@@ -128,6 +146,15 @@ vs_status vs_set_color(vs_session *s, const uint8_t *rgb_planar, int is_host, vs
 /* A2-A3: bilateral filter of the raw frame into pyramid level 0 (metres).
  * Replaces LGKvImageProcessor::bfd_Bilateral_Filter_Depth (z_yooji_cuda_img_proc.cu:394-417). */
 vs_status vs_bilateral(vs_session *s, vs_stream stream);
+/* A2-A3 with the reference operator's buffers
+ * (LGKvImageProcessor::bfd_Bilateral_Filter_Depth(float *dev_out, const float *host_in, w, h),
+ * z_yooji_cuda_img_proc.cu:394-417, and its _Host variant :420-442): depth_in is
+ * a frame in the session's depth units, on the host when in_is_host, else on
+ * the device; NULL filters the frame already in the session's raw buffer. The
+ * result goes to pyramid level 0 and, when depth_out_dev is not NULL, also to
+ * that device buffer of width*height floats (metres). */
+vs_status vs_bilateral_filter_depth(vs_session *s, const void *depth_in, int in_is_host, float *depth_out_dev,
+                                    vs_stream stream);
 /* A4: levels 1..L-1 from level 0 (5x5 binomial blur + decimation), as
  * cpdd_Construct_Pyramids_Down_Depth (z_yooji_image_processor.cpp:368-407). */
 vs_status vs_pyramid(vs_session *s, vs_stream stream);
@@ -148,9 +175,24 @@ vs_status vs_track(vs_session *s, const double *T_01_init, vs_stream stream);
  * (z_yooji_cuda_volume_integrator.cu:936-972) with the per-frame brick-mask
  * clear of cdtocc (:1006-1010). */
 vs_status vs_integrate(vs_session *s, vs_stream stream);
+/* A13-A15 on a caller's device frame, as cdtoc_Convert_Depth_to_TSDF_on_Cube(cube,
+ * const float *depth_dev, w, h, T_gc_dev, T_cg_dev) (z_yooji_cuda_volume_integrator.cu:936-972):
+ * depth_dev (session's size and depth units; NULL = the raw buffer) is copied
+ * into the raw buffer, then integrated at the session's current pose. */
+vs_status vs_integrate_depth(vs_session *s, const void *depth_dev, vs_stream stream);
 /* A16-A18: raycast the volume at `level` into the session's model maps.
  * Replaces LGKvRendererTSDF::rmi_Render_Maps_for_ICP (z_yooji_cuda_tsdf_renderer.cu:1642-1700). */
 vs_status vs_raycast(vs_session *s, int level, vs_stream stream);
+/* A16-A18 into caller-owned device buffers, as rmi_Render_Maps_for_ICP(cube, w, h,
+ * T_gc_dev, T_cg_dev, cam_cen, light, level, io_map_depth, io_map_vertex,
+ * io_map_normals, io_img_normals) (z_yooji_cuda_tsdf_renderer.cu:1642-1700): raycasts
+ * `level` at the session's current pose into its model maps (which the next
+ * track reads) and writes them to the buffers given (each may be NULL):
+ * depth W_l*H_l floats (camera z, 0 = miss), vertex and normal W_l*H_l*3
+ * floats (camera frame, -100 = miss), shaded normal image W_l*H_l bytes.
+ * Asynchronous on `stream`. */
+vs_status vs_render_maps(vs_session *s, int level, float *depth_dev, float *vtx_dev, float *nrm_dev,
+                         uint8_t *img_dev, vs_stream stream);
 /* A19: one fused frame (filter -> pyramid -> track (frames > 0) -> integrate
  * raw depth -> raycast every level), as so_Scan_Object
  * (z_yooji_object_scanner.cpp:472-629). Asynchronous; see vs_get_frame_status.
@@ -168,9 +210,9 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
  * The CPU steps either side of the filter in the reference's input path, on
  * the device. Depth buffers are device pointers: uint16 millimetres, or float
  * metres when *_is_float. A NULL frame pointer selects the session's own
- * raw-frame buffer (uint16, the session's size and intrinsics), which a
- * following vs_fuse_frame / vs_fuse_batch / vs_server_step with a NULL depth
- * then fuses as it is. */
+ * raw-frame buffer (the session's size, intrinsics and depth units, which
+ * *_is_float must then match), which a following vs_fuse_frame /
+ * vs_fuse_batch / vs_server_step with a NULL depth fuses as it is. */
 /* z_tdrcc_Transform_Depth_to_Rgb_Camera_Coordinates (z_yooji_object_scanner.cpp:2429-2476, with
  * CKvYooji_MatrixRgbD::convert_pixel_coord, z_yooji_matrix_rgbd.cpp:101-118): forward-project the
  * source frame (src_w x src_h, intrinsics K_src) through T_dst_src (row-major, source camera to the
@@ -254,10 +296,17 @@ vs_status vs_ctx_set_tuning(vs_ctx *ctx, int mode, const char *cache_path);
 vs_status vs_ctx_tuning(vs_ctx *ctx, char *buf, size_t len, size_t *needed);
 /* Counts from the last integrate: visible bricks and updated voxels. */
 vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long long *updated_voxels);
-/* Tap counts of one instrumented raycast at `level` with the current pose and
- * volume: nearest taps, trilinear evaluations, hits (synchronous). They price
- * the raycast's algorithmic bytes; no counterpart in the reference. */
-vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]);
+/* Tap statistics of one instrumented raycast at `level` with the current pose
+ * and volume, marching every step as the reference does (synchronous):
+ *   [0] nearest taps (d_gtvu), [1] trilinear evaluations (d_gtvi), [2] hits,
+ *   [3] nearest taps outside the grid, [4] nearest taps of voxels without weight,
+ *   [5] bytes the nearest taps request, [6] bytes the trilinear evaluations
+ *   request, [7] bytes the hits' 32-tap normals (d_csnt) request.
+ * Bytes follow the reference's lookups (z_yooji_cuda_tsdf_renderer.cu:68-153,288-464):
+ * nothing outside the grid, the weight alone for a voxel without weight,
+ * weight + value otherwise, and a stencil stops at its first failing tap.
+ * They price the raycast's algorithmic bytes; no counterpart in the reference. */
+vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[8]);
 
 /* Volume carving: g_uvv_Update_Valid_Volume (z_yooji_cuda_volume_integrator.cu:718-799,
  * LGKvVolumeIntegrator::uvv_Update_Valid_Volume :1035-1060). valid = one byte per voxel in the
@@ -282,7 +331,7 @@ vs_status vs_volume_carve(vs_session *s, const uint8_t *valid, int is_host, unsi
  * re-rendered at the saved pose. */
 typedef struct {
     char magic[8];           /* "VSCANSNP" */
-    uint32_t version;        /* 1 */
+    uint32_t version;        /* 2 */
     uint32_t header_bytes;   /* sizeof(vs_snapshot_header) */
     vs_session_desc desc;
     double T_gc[16];

@@ -416,7 +416,9 @@ long long icp_system(const float *d1, int W, int H, const float *K, const float 
 // ---------------------------------------------------------------------------
 // A10 Cholesky + solve (_yooji_cuda_math.h:489-592), A11 pose algebra
 // ---------------------------------------------------------------------------
-bool cholesky_solve(const double *A, const double *b, double *x) {
+// rel > 0: also fail (status 8, the build's relative pivot guard) when a
+// squared pivot L_jj^2 < rel * A_jj. Returns 0, 4 (absolute pivot) or 8.
+int cholesky_status(const double *A, const double *b, double rel, double *x) {
     double L[36] = {0};
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j <= i; ++j) {
@@ -433,7 +435,10 @@ bool cholesky_solve(const double *A, const double *b, double *x) {
             }
         }
     for (int i = 0; i < 6; ++i)
-        if (!(std::fabs(L[i * 6 + i]) >= 1e-8)) return false;
+        if (!(std::fabs(L[i * 6 + i]) >= 1e-8)) return 4;
+    if (rel > 0.0)
+        for (int i = 0; i < 6; ++i)
+            if (L[i * 6 + i] * L[i * 6 + i] < rel * A[i * 6 + i]) return 8;
     double y[6];
     for (int i = 0; i < 6; ++i) {
         double s = 0;
@@ -445,8 +450,9 @@ bool cholesky_solve(const double *A, const double *b, double *x) {
         for (int k = i + 1; k < 6; ++k) s += L[k * 6 + i] * x[k];
         x[i] = (y[i] - s) / L[i * 6 + i];
     }
-    return true;
+    return 0;
 }
+bool cholesky_solve(const double *A, const double *b, double *x) { return cholesky_status(A, b, 0.0, x) == 0; }
 
 // SE(3) exponential of the twist x = [w; v] (Ma et al., "An Invitation to 3D
 // Vision", eq. 2.30, as used by z_uitme_..._Matrix_Exponential,
@@ -530,7 +536,7 @@ double rotation_deg(const double *T) {
 template <class R>
 int icp_track(const float *const *d1, int W0, int H0, const float *K, int levels, const int *iters,
               const float *const *mv, const float *const *mn, float dist2, int min_valid, float max_m,
-              float max_deg, const double *T_init, double *T_gc, int *iters_done) {
+              float max_deg, float rel_pivot, const double *T_init, double *T_gc, int *iters_done) {
     double T12[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}, T21[16];
     std::memcpy(T21, T12, sizeof(T12));
     int nlev = levels;
@@ -555,9 +561,10 @@ int icp_track(const float *const *d1, int W0, int H0, const float *K, int levels
             double ATA[36], ATb[6], x[6];
             for (int k = 0; k < 36; ++k) ATA[k] = ATAr[k];
             for (int k = 0; k < 6; ++k) ATb[k] = ATbr[k];
-            if (!cholesky_solve(ATA, ATb, x)) {
+            const int cs = cholesky_status(ATA, ATb, (double)rel_pivot, x);
+            if (cs != 0) {
                 if (iters_done) *iters_done = solves;
-                return 4;
+                return cs;
             }
             double ninf = 0;
             for (int k = 0; k < 6; ++k) ninf = std::max(ninf, std::fabs(x[k]));
@@ -1183,7 +1190,7 @@ int fuse_frame_impl(or_session *s, const uint16_t *mm, double *ms) {
             mn[l] = s->mnrm[l].data();
         }
         status = icp_track<R>(dp, W, H, s->K, d.levels, d.iters, mv, mn, d.icp_dist2, d.icp_min_valid,
-                              d.max_motion_m, d.max_motion_deg, nullptr, s->Tgc, nullptr);
+                              d.max_motion_m, d.max_motion_deg, d.icp_rel_pivot, nullptr, s->Tgc, nullptr);
     }
     auto t3 = clk::now();
     if (status != 0) {
@@ -1275,9 +1282,12 @@ void or_rigid_inverse(const double T[16], double Ti[16]) { rigid_inverse(T, Ti);
 void or_matmul4(const double A[16], const double B[16], double C[16]) { matmul4(A, B, C); }
 int or_icp_track_d(const float *const *d1, int W0, int H0, const float *K, int levels, const int iters[3],
                    const float *const *mv, const float *const *mn, float dist2, int min_valid, float max_m,
-                   float max_deg, const double *T_init, double T_gc[16], int *iters_done) {
-    return icp_track<double>(d1, W0, H0, K, levels, iters, mv, mn, dist2, min_valid, max_m, max_deg, T_init, T_gc,
-                             iters_done);
+                   float max_deg, float rel_pivot, const double *T_init, double T_gc[16], int *iters_done) {
+    return icp_track<double>(d1, W0, H0, K, levels, iters, mv, mn, dist2, min_valid, max_m, max_deg, rel_pivot,
+                             T_init, T_gc, iters_done);
+}
+int or_cholesky_status(const double A[36], const double b[6], double rel, double x[6]) {
+    return cholesky_status(A, b, rel, x);
 }
 void or_integrate_d(const uint16_t *mm, int W, int H, const float K[4], const float Tgc[16], const float Tcg[16],
                     const or_volume *vol, double *F, double *Wt, uint8_t *bm, uint8_t *upd, long long counts[2]) {

@@ -59,6 +59,9 @@ long long or_icp_system_d(const float *depth1, int W, int H, const float K[4], c
 
 /* A10: reference Cholesky + solve semantics in FP64; returns 1 on success. */
 int or_cholesky_solve(const double ATA[36], const double ATb[6], double x[6]);
+/* The same with the build's relative pivot guard (rel > 0): returns 0 (solved),
+ * 4 (a pivot below 1e-8, the reference's failure) or 8 (L_jj^2 < rel * A_jj). */
+int or_cholesky_status(const double ATA[36], const double ATb[6], double rel, double x[6]);
 /* A11: SE(3) exponential of x = [w; v] (FP64), rigid inverse, 4x4 product. */
 void or_expmap(const double x[6], double T[16]);
 void or_rigid_inverse(const double T[16], double Ti[16]);
@@ -66,11 +69,12 @@ void or_matmul4(const double A[16], const double B[16], double C[16]);
 
 /* A12: coarse-to-fine tracking. depth1[l], mvtx[l], mnrm[l] per level; K has
  * 4 floats per level. T_gc (in/out). Returns a vs_status code (0 ok, 4 lost,
- * 5 motion rejected, 7 too few points). iters_done (nullable) = total solves. */
+ * 5 motion rejected, 7 too few points, 8 relative pivot guard). iters_done
+ * (nullable) = total solves. rel_pivot as or_cholesky_status (0 = off). */
 int or_icp_track_d(const float *const *depth1, int W0, int H0, const float *K, int levels, const int iters[3],
                    const float *const *mvtx, const float *const *mnrm, float dist2, int min_valid,
-                   float max_motion_m, float max_motion_deg, const double *T_init_01, double T_gc[16],
-                   int *iters_done);
+                   float max_motion_m, float max_motion_deg, float rel_pivot, const double *T_init_01,
+                   double T_gc[16], int *iters_done);
 
 /* A13-A14: brick visibility + TSDF update. brick_mask and upd_mask (nullable)
  * are outputs; counts[0] = visible bricks, counts[1] = updated voxels. */
@@ -130,6 +134,7 @@ typedef struct {
     float bil_sigma_s, bil_sigma_r;
     int bil_radius;
     int fp32; /* 1 = FP32/OpenMP baseline instantiation, 0 = FP64 oracle */
+    float icp_rel_pivot; /* relative pivot guard of the solve, 0 = off */
 } or_session_desc;
 or_session *or_session_create(const or_session_desc *d, const double T_gc0[16]);
 void or_session_destroy(or_session *s);

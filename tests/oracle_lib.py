@@ -31,7 +31,7 @@ class OrSessionDesc(C.Structure):
         ("max_motion_m", C.c_float), ("max_motion_deg", C.c_float),
         ("vol", OrVolume),
         ("bil_sigma_s", C.c_float), ("bil_sigma_r", C.c_float), ("bil_radius", C.c_int),
-        ("fp32", C.c_int),
+        ("fp32", C.c_int), ("icp_rel_pivot", C.c_float),
     ]
 
 
@@ -81,7 +81,9 @@ def _load_oracle(path):
     L.or_rigid_inverse.argtypes = [P, P]
     L.or_matmul4.argtypes = [P, P, P]
     L.or_icp_track_d.argtypes = [P, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_float, C.c_int, C.c_float,
-                                 C.c_float, P, P, P]
+                                 C.c_float, C.c_float, P, P, P]
+    L.or_cholesky_status.argtypes = [P, P, C.c_double, P]
+    L.or_cholesky_status.restype = C.c_int
     L.or_icp_track_d.restype = C.c_int
     L.or_integrate_d.argtypes = [P, C.c_int, C.c_int, P, P, P, C.POINTER(OrVolume), P, P, P, P, P]
     L.or_integrate_f.argtypes = [P, C.c_int, C.c_int, P, P, P, C.POINTER(OrVolume), P, P, P, P, P]
@@ -191,6 +193,14 @@ def icp_system(depth1, K, mvtx, mnrm, T21, dist2):
     return ATA.reshape(6, 6), ATb, n, mask
 
 
+def cholesky_status(A, b, rel):
+    A = np.ascontiguousarray(A, np.float64).reshape(36)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(6)
+    st = oracle().or_cholesky_status(_p(A), _p(b), float(rel), _p(x))
+    return st, x
+
+
 def cholesky_solve(A, b):
     A = np.ascontiguousarray(A, np.float64).reshape(36)
     b = np.ascontiguousarray(b, np.float64)
@@ -213,7 +223,8 @@ def rigid_inverse(T):
     return o.reshape(4, 4)
 
 
-def icp_track(depths, K, levels, iters, mvtx, mnrm, dist2, T_gc, min_valid=0, max_m=0.0, max_deg=0.0, T_init=None):
+def icp_track(depths, K, levels, iters, mvtx, mnrm, dist2, T_gc, min_valid=0, max_m=0.0, max_deg=0.0, T_init=None,
+              rel_pivot=0.0):
     d = [np.ascontiguousarray(x, np.float32) for x in depths]
     mv = [np.ascontiguousarray(x, np.float32) for x in mvtx]
     mn = [np.ascontiguousarray(x, np.float32) for x in mnrm]
@@ -226,7 +237,7 @@ def icp_track(depths, K, levels, iters, mvtx, mnrm, dist2, T_gc, min_valid=0, ma
     Ti = None if T_init is None else np.ascontiguousarray(T_init, np.float64).reshape(16)
     nd = C.c_int()
     st = oracle().or_icp_track_d(dp, W0, H0, _p(np.ascontiguousarray(K, np.float32)), levels, it, mvp, mnp, dist2,
-                                 min_valid, max_m, max_deg, _p(Ti), _p(T), C.byref(nd))
+                                 min_valid, max_m, max_deg, rel_pivot, _p(Ti), _p(T), C.byref(nd))
     return st, T.reshape(4, 4), nd.value
 
 
@@ -327,7 +338,7 @@ class OracleSession:
     def __init__(self, desc, T_gc0, fp32=False):
         d = OrSessionDesc()
         for f in ["width", "height", "fx", "fy", "cx", "cy", "levels", "icp_dist2", "icp_min_valid",
-                  "max_motion_m", "max_motion_deg", "bil_sigma_s", "bil_sigma_r", "bil_radius"]:
+                  "max_motion_m", "max_motion_deg", "bil_sigma_s", "bil_sigma_r", "bil_radius", "icp_rel_pivot"]:
             setattr(d, f, getattr(desc, f))
         for k in range(3):
             d.iters[k] = desc.iters[k]

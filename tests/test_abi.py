@@ -53,6 +53,16 @@ def test_session_bytes_and_descriptor_checks():
     bad = vscan.make_desc(w)
     bad.levels = 4
     assert vscan.session_bytes(bad) == 0
+    bad = vscan.make_desc(w)
+    bad.depth_units = 2
+    assert vscan.session_bytes(bad) == 0
+    for rel in (-1e-3, 1.0, float("nan")):
+        bad = vscan.make_desc(w)
+        bad.icp_rel_pivot = rel
+        assert vscan.session_bytes(bad) == 0
+    # a float-metre session's raw frame is 4 bytes a pixel instead of 2
+    f = vscan.make_desc(w, depth_units=vscan.VS_DEPTH_M_F32)
+    assert vscan.session_bytes(f) == nb + 640 * 480 * 2
 
 
 def test_default_descriptor_matches_reference_defaults():

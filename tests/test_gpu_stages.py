@@ -34,18 +34,25 @@ def sess(ctx, stream):
     s.close()
 
 
-def test_bilateral_parity(sess, stream):
-    print("case: bilateral 320x240 frames 0,7", flush=True)
-    d = sess.desc
-    for fr in (0, 7):
-        mm = stream.frame(fr)
-        sess.set_depth(mm)
-        sess.bilateral()
-        got = sess.filtered(0)
-        ref = orc.bilateral(mm, d.bil_sigma_s, d.bil_sigma_r, d.bil_radius)
-        assert np.array_equal(got > 0, ref > 0), "validity mask differs"
-        err = np.abs(got.astype(np.float64) - ref)
-        assert err.max() <= 1e-6, f"max filtered-depth error {err.max() * 1e3:.3g} mm > 1e-3 mm"
+@pytest.mark.parametrize("radius,sigma_s", [(1, 1.5), (2, 4.0), (3, 2.5)])
+def test_bilateral_parity(ctx, stream, radius, sigma_s):
+    print(f"case: bilateral 320x240 frames 0,7, radius {radius}", flush=True)
+    d = vscan.make_desc(W0, K=stream.K)
+    d.bil_radius = radius
+    d.bil_sigma_s = sigma_s
+    sess = vscan.Session(ctx, d, T_gc0=stream.pose(0))
+    try:
+        for fr in (0, 7):
+            mm = stream.frame(fr)
+            sess.set_depth(mm)
+            sess.bilateral()
+            got = sess.filtered(0)
+            ref = orc.bilateral(mm, d.bil_sigma_s, d.bil_sigma_r, d.bil_radius)
+            assert np.array_equal(got > 0, ref > 0), "validity mask differs"
+            err = np.abs(got.astype(np.float64) - ref)
+            assert err.max() <= 1e-6, f"max filtered-depth error {err.max() * 1e3:.3g} mm > 1e-3 mm"
+    finally:
+        sess.close()
 
 
 def test_bilateral_640(ctx):

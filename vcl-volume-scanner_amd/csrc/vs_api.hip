@@ -96,7 +96,7 @@ Layout make_layout(const vs_session_desc &d) {
     const size_t sv = d.tsdf_type == VS_TSDF_F16 ? 2 : 4, sw = d.weight_type == VS_WEIGHT_U8 ? 1 : 4;
     L.state = take(sizeof(SessState));
     L.params = take(sizeof(SessParams));
-    L.raw = take((size_t)W * H * 2);
+    L.raw = take((size_t)W * H * (d.depth_units == VS_DEPTH_M_F32 ? 4 : 2));
     for (int l = 0; l < kMaxLevels; ++l) {
         const int Wl = W >> l, Hl = H >> l;
         const size_t n = (size_t)Wl * Hl;
@@ -155,7 +155,16 @@ vs_status check_desc(const vs_session_desc *d) {
     if (d->bil_radius < 1 || d->bil_radius > 3 || !(d->bil_sigma_s > 0) || !(d->bil_sigma_r > 0))
         return fail(VS_EINVAL, "bilateral parameters");
     if (!(d->fx > 0) || !(d->fy > 0)) return fail(VS_EINVAL, "focal length");
+    if (d->depth_units != VS_DEPTH_MM_U16 && d->depth_units != VS_DEPTH_M_F32)
+        return fail(VS_EINVAL, "depth_units %d", d->depth_units);
+    if (!(d->icp_rel_pivot >= 0.0f && d->icp_rel_pivot < 1.0f))
+        return fail(VS_EINVAL, "icp_rel_pivot must be in [0, 1)");
     return VS_OK;
+}
+
+// Bytes of one depth frame of the session's size in its units.
+size_t frame_bytes(const vs_session_desc &d) {
+    return (size_t)d.width * d.height * (d.depth_units == VS_DEPTH_M_F32 ? 4 : 2);
 }
 
 }  // namespace
@@ -723,9 +732,9 @@ vs_status sync_last(vs_session *s) {
 }
 
 // A NULL frame keeps the raw buffer as it is (set by vs_set_depth or a pre-step).
-vs_status copy_depth(vs_session *s, const uint16_t *depth, int is_host, hipStream_t st) {
+vs_status copy_depth(vs_session *s, const void *depth, int is_host, hipStream_t st) {
     if (!depth) return VS_OK;
-    const size_t bytes = (size_t)s->d.width * s->d.height * 2;
+    const size_t bytes = frame_bytes(s->d);
     HIP_TRY(hipMemcpyAsync((void *)s->hp.raw, depth, bytes, is_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                            st));
     return VS_OK;
@@ -745,6 +754,7 @@ const char *vs_status_string(int st) {
     case VS_MOTION_REJECTED: return "motion rejected";
     case VS_BUDGET: return "HBM budget exhausted";
     case VS_TOO_FEW_POINTS: return "too few correspondences";
+    case VS_SINGULAR: return "singular normal equations (relative pivot guard)";
     default: return "unknown status";
     }
 }
@@ -772,6 +782,9 @@ void vs_session_desc_default(vs_session_desc *d, int width, int height, int n, f
     d->bil_sigma_s = 4.0f;
     d->bil_sigma_r = 0.1f;
     d->bil_radius = 2;
+    d->integrate_mode = VS_INTEGRATE_DEPTH;
+    d->depth_units = VS_DEPTH_MM_U16;
+    d->icp_rel_pivot = 0.0f;
 }
 
 vs_status vs_ctx_create(int device, vs_ctx **out) {
@@ -849,6 +862,8 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     P.icp_min_valid = d.icp_min_valid;
     P.max_motion_m = d.max_motion_m;
     P.max_motion_deg = d.max_motion_deg;
+    P.icp_rel_pivot = d.icp_rel_pivot;
+    P.raw_f32 = d.depth_units == VS_DEPTH_M_F32;
     for (int k = 0; k < 3; ++k) {
         P.vdim[k] = d.vol_dim[k];
         P.bdim[k] = d.vol_dim[k] / kBrick;
@@ -879,7 +894,7 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     }
     P.bil_inv2s2 = 1.0f / (2.0f * d.bil_sigma_r * d.bil_sigma_r);
     P.bil_radius = d.bil_radius;
-    P.raw = (const uint16_t *)(m + L.raw);
+    P.raw = m + L.raw;
     for (int l = 0; l < kMaxLevels; ++l) {
         P.filt[l] = (float *)(m + L.filt[l]);
         P.mvtx[l] = (MapXYZ *)(m + L.mvtx[l]);
@@ -1000,7 +1015,8 @@ vs_status vs_depth_reproject(vs_session *s, const void *src, int src_w, int src_
                              const float T_dst_src[16], int src_is_float, void *dst, vs_stream stream) {
     if (!s || !src || !K_src || !T_dst_src) return fail(VS_EINVAL, "null argument");
     if (src_w < 1 || src_h < 1 || src_w > 8192 || src_h > 8192) return fail(VS_EINVAL, "source size %dx%d", src_w, src_h);
-    if (!dst && src_is_float) return fail(VS_EINVAL, "the session's raw buffer holds uint16 millimetres");
+    if (!dst && (src_is_float != 0) != (s->d.depth_units == VS_DEPTH_M_F32))
+        return fail(VS_EINVAL, "src_is_float does not match the depth units of the session's raw buffer");
     if (!(K_src[0] != 0.0f) || !(K_src[1] != 0.0f)) return fail(VS_EINVAL, "source focal length");
     const float *Kd = s->hp.lev[0].K;
     launch_depth_reproject(src_is_float, src, src_w, src_h, K_src, Kd, T_dst_src,
@@ -1014,7 +1030,8 @@ vs_status vs_depth_crop_to_volume(vs_session *s, void *depth, int w, int h, cons
                                   int is_float, vs_stream stream) {
     if (!s) return fail(VS_EINVAL, "null session");
     if (!depth) {
-        if (is_float) return fail(VS_EINVAL, "the session's raw buffer holds uint16 millimetres");
+        if ((is_float != 0) != (s->d.depth_units == VS_DEPTH_M_F32))
+            return fail(VS_EINVAL, "is_float does not match the depth units of the session's raw buffer");
         depth = (void *)s->hp.raw;
         w = s->d.width;
         h = s->d.height;
@@ -1037,6 +1054,21 @@ vs_status vs_bilateral(vs_session *s, vs_stream stream) {
     launch_fuse_ok(s->self.dP, 1, h);
     launch_bilateral(s->self.dP, s->self.dWork[0], s->self.nWork[0], h);
     HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_bilateral_filter_depth(vs_session *s, const void *depth_in, int in_is_host, float *depth_out_dev,
+                                    vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    hipStream_t h = pick(s, stream);
+    vs_status r = copy_depth(s, depth_in, in_is_host, h);
+    if (r != VS_OK) return r;
+    launch_fuse_ok(s->self.dP, 1, h);
+    launch_bilateral(s->self.dP, s->self.dWork[0], s->self.nWork[0], h);
+    HIP_TRY(hipGetLastError());
+    if (depth_out_dev)
+        HIP_TRY(hipMemcpyAsync(depth_out_dev, s->hp.filt[0], (size_t)s->d.width * s->d.height * 4,
+                               hipMemcpyDeviceToDevice, h));
     return VS_OK;
 }
 
@@ -1078,6 +1110,24 @@ vs_status vs_integrate(vs_session *s, vs_stream stream) {
     launch_vis_clear(B.dP, 1, B.bl, h);
     launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, h);
     launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), B.sd_slices, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_integrate_depth(vs_session *s, const void *depth_dev, vs_stream stream) {
+    if (!s) return fail(VS_EINVAL, "null session");
+    vs_status r = copy_depth(s, depth_dev, 0, pick(s, stream));
+    if (r != VS_OK) return r;
+    return vs_integrate(s, stream);
+}
+
+vs_status vs_render_maps(vs_session *s, int level, float *depth_dev, float *vtx_dev, float *nrm_dev,
+                         uint8_t *img_dev, vs_stream stream) {
+    vs_status r = vs_raycast(s, level, stream);
+    if (r != VS_OK) return r;
+    const size_t npx = (size_t)(s->d.width >> level) * (s->d.height >> level);
+    if (depth_dev || vtx_dev || nrm_dev || img_dev)
+        launch_maps_out(s->self.dP, level, npx, depth_dev, vtx_dev, nrm_dev, img_dev, s->last);
     HIP_TRY(hipGetLastError());
     return VS_OK;
 }
@@ -1379,7 +1429,7 @@ vs_status vs_session_snapshot(vs_session *s, void *buf, size_t capacity, size_t 
     vs_snapshot_header h;
     std::memset(&h, 0, sizeof(h));
     std::memcpy(h.magic, "VSCANSNP", 8);
-    h.version = 1;
+    h.version = 2;
     h.header_bytes = sizeof(h);
     h.desc = s->d;
     std::memcpy(h.T_gc, st.T_gc, sizeof(h.T_gc));
@@ -1413,8 +1463,8 @@ vs_status vs_session_snapshot(vs_session *s, void *buf, size_t capacity, size_t 
 static vs_status snapshot_check(const void *buf, size_t bytes, vs_snapshot_header &h) {
     if (!buf || bytes < sizeof(vs_snapshot_header)) return fail(VS_EINVAL, "snapshot too short");
     std::memcpy(&h, buf, sizeof(h));
-    if (std::memcmp(h.magic, "VSCANSNP", 8) != 0 || h.version != 1 || h.header_bytes != sizeof(h))
-        return fail(VS_EINVAL, "not a version-1 snapshot");
+    if (std::memcmp(h.magic, "VSCANSNP", 8) != 0 || h.version != 2 || h.header_bytes != sizeof(h))
+        return fail(VS_EINVAL, "not a version-2 snapshot");
     vs_status cs = check_desc(&h.desc);
     if (cs != VS_OK) return cs;
     uint32_t sv, sw, sc;
@@ -1657,22 +1707,20 @@ vs_status vs_get_integrate_counts(vs_session *s, long long *visible_bricks, long
     return VS_OK;
 }
 
-/* Tap counts of one instrumented raycast at `level` with the current pose:
- * [nearest taps, trilinear evaluations, hits]. Used to compute the algorithmic
- * bytes of the raycast; not part of the reference surface. */
-vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[3]) {
+// Tap statistics of one instrumented raycast (layout in vscan.h).
+vs_status vs_raycast_counts(vs_session *s, int level, unsigned long long counts[8]) {
     if (!s || !counts) return fail(VS_EINVAL, "null argument");
     if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
     vs_status r = sync_last(s);
     if (r != VS_OK) return r;
     unsigned long long *d = (unsigned long long *)(s->mem + s->L.stats);
-    HIP_TRY(hipMemset(d, 0, 32));
+    HIP_TRY(hipMemset(d, 0, 64));
     const Batch &B = s->self;
     launch_fuse_ok(B.dP, 1, s->stream);
     launch_raycast_stats(B.dP, B.dWork[level], B.nWork[level], level, B.tt, B.wt, d, s->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(counts, d, 24, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(counts, d, 64, hipMemcpyDeviceToHost));
     return VS_OK;
 }
 

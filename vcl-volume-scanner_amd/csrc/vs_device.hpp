@@ -90,6 +90,8 @@ struct alignas(16) SessParams {
     float icp_dist2;
     int icp_min_valid;
     float max_motion_m, max_motion_deg;
+    float icp_rel_pivot;  // relative Cholesky pivot guard (0 = off)
+    int raw_f32;          // raw frame holds float metres (else uint16 mm)
     int vdim[3];        // voxels per axis
     int bdim[3];        // bricks per axis
     float origin[3];
@@ -103,7 +105,7 @@ struct alignas(16) SessParams {
     float bil_inv2s2;   // 1 / (2 sigma_r^2)
     int bil_radius;
     // buffers (device)
-    const uint16_t *raw;     // W*H depth (mm), level-0 size
+    const void *raw;         // W*H depth, level-0 size: uint16 mm, or float m when raw_f32
     float *filt[kMaxLevels];
     MapXYZ *mvtx[kMaxLevels];  // model vertex, -100 invalid
     MapXYZ *mnrm[kMaxLevels];  // model normal, -100 invalid
@@ -236,6 +238,16 @@ __device__ __forceinline__ M34 ld_m34(const float *T) {
 }
 
 __device__ __forceinline__ float depth_m(uint16_t mm) { return (float)mm * 0.001f; }
+// Depth in metres of raw-frame pixel i: uint16 millimetres * 0.001f (0 stays
+// invalid), or a float-metre frame, where any value that is not a finite
+// positive number (0, negative, NaN, infinite) reads as 0, i.e. invalid.
+__device__ __forceinline__ float raw_depth(const SessParams &P, size_t i) {
+    if (P.raw_f32) {
+        const float v = ldg((const float *)P.raw, i);
+        return (v > 0.0f && v <= 3.402823466e38f) ? v : 0.0f;
+    }
+    return depth_m(ldg((const uint16_t *)P.raw, i));
+}
 // __float2int_rn (round half to even); non-finite or |v| >= 2^31 is out of range
 __device__ __forceinline__ bool rint_ref(float v, int &out) {
     if (!(v > -2147483520.0f && v < 2147483520.0f)) return false;

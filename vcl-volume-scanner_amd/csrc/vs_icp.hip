@@ -81,8 +81,14 @@ __device__ void expmap(const double *x, double *T) {
 // Cholesky and triangular solves with the reference's pivot rule
 // (d_lld_LL_Decomposition / d_sls_Solve_Linear_System_using_LLD,
 // _yooji_cuda_math.h:489-592): entries below a pivot < 1e-8 are zeroed and the
-// solve fails if any pivot is below 1e-8 (or not a number).
-__device__ bool cholesky_solve(const double *A, const double *b, double *x) {
+// solve fails (status 4) if any pivot is below 1e-8 (or not a number). With
+// rel > 0 it also fails (status 8) when a squared pivot L_jj^2 is below
+// rel * A_jj: column j of J is then nearly a combination of the columns
+// before it (a plane leaves 3 of the 6 motions unconstrained), a near-rank
+// deficiency the absolute test cannot see at FP32 scales. The ratio is
+// invariant to the units of the rotation and translation columns.
+// Returns 0 on success.
+__device__ int cholesky_solve(const double *A, const double *b, double rel, double *x) {
     double L[36];
     for (int k = 0; k < 36; ++k) L[k] = 0.0;
     for (int i = 0; i < 6; ++i)
@@ -100,7 +106,10 @@ __device__ bool cholesky_solve(const double *A, const double *b, double *x) {
             }
         }
     for (int i = 0; i < 6; ++i)
-        if (!(fabs(L[i * 6 + i]) >= 1e-8)) return false;
+        if (!(fabs(L[i * 6 + i]) >= 1e-8)) return 4;
+    if (rel > 0.0)
+        for (int i = 0; i < 6; ++i)
+            if (L[i * 6 + i] * L[i * 6 + i] < rel * A[i * 6 + i]) return 8;
     double y[6];
     for (int i = 0; i < 6; ++i) {
         double s = 0.0;
@@ -112,7 +121,7 @@ __device__ bool cholesky_solve(const double *A, const double *b, double *x) {
         for (int k = i + 1; k < 6; ++k) s += L[k * 6 + i] * x[k];
         x[i] = (y[i] - s) / L[i * 6 + i];
     }
-    return true;
+    return 0;
 }
 
 __device__ void publish_T21(SessState *st) {
@@ -348,8 +357,9 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
         stg(&st->icp_failed, 0, 7);
         return;
     }
-    if (!cholesky_solve(ATA, ATb, x6)) {
-        stg(&st->icp_failed, 0, 4);
+    const int cs = cholesky_solve(ATA, ATb, (double)P.icp_rel_pivot, x6);
+    if (cs != 0) {
+        stg(&st->icp_failed, 0, cs);
         return;
     }
     double ninf = 0.0;

@@ -13,13 +13,13 @@ namespace vs {
 constexpr int kHalo = 3;  // largest bilateral radius supported
 
 // The 2R+1 x 2R+1 window of one pixel (tile coordinates of the centre): sum
-// of w and of w (d - c), and whether a tap is <= 0. RR > 0: the radius at
-// compile time (the loops unroll); RR = 0: R at run time. Same operations in
-// the same order either way.
-template <int RR, int TW>
-__device__ __forceinline__ void bil_window(const float (*tile)[TW], int cy, int cx, int R, const float *g, float c,
+// of w and of w (d - c), and whether a tap is <= 0. The radius is a
+// compile-time constant (1, 2 or 3, the radii a session accepts), so the
+// loops unroll.
+template <int R, int TW>
+__device__ __forceinline__ void bil_window(const float (*tile)[TW], int cy, int cx, const float *g, float c,
                                            float inv2s2, float &s, float &t, bool &zero) {
-    const int r = RR > 0 ? RR : R;
+    constexpr int r = R;
 #pragma unroll
     for (int i = -r; i <= r; ++i) {
         const float gi = g[i + r];
@@ -51,13 +51,12 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
     const int W = L.W, H = L.H;
     const int x0 = (wi.tile % L.tiles_x) * kTileW, y0 = (wi.tile / L.tiles_x) * kTileH;
     const int R = P.bil_radius;
-    const uint16_t *raw = P.raw;
     constexpr int TW = kTileW + 2 * kHalo, TH = kTileH + 2 * kHalo;
     for (int i = threadIdx.x; i < TW * TH; i += kTileThreads) {
         int ly = i / TW, lx = i - ly * TW;
         int gx = min(max(x0 + lx - kHalo, 0), W - 1);
         int gy = min(max(y0 + ly - kHalo, 0), H - 1);
-        tile[ly][lx] = depth_m(ldg(raw, (size_t)gy * W + gx));
+        tile[ly][lx] = raw_depth(P, (size_t)gy * W + gx);
     }
     __syncthreads();
     const int lx = threadIdx.x % kTileW, ly = threadIdx.x / kTileW;
@@ -71,14 +70,13 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
     for (int k = 0; k < 2 * kHalo + 1; ++k) g[k] = P.bil_g[k];
     const float inv2s2 = P.bil_inv2s2;
     const int cy = ly + kHalo, cx = lx + kHalo;
+    // session creation accepts radii 1..3 only
     if (R == 2)
-        bil_window<2, TW>(tile, cy, cx, R, g, c, inv2s2, s, t, zero);
+        bil_window<2, TW>(tile, cy, cx, g, c, inv2s2, s, t, zero);
     else if (R == 1)
-        bil_window<1, TW>(tile, cy, cx, R, g, c, inv2s2, s, t, zero);
-    else if (R == 3)
-        bil_window<3, TW>(tile, cy, cx, R, g, c, inv2s2, s, t, zero);
-    else  // run-time radius: weights read where they live (no dynamic register index)
-        bil_window<0, TW>(tile, cy, cx, R, P.bil_g, c, inv2s2, s, t, zero);
+        bil_window<1, TW>(tile, cy, cx, g, c, inv2s2, s, t, zero);
+    else
+        bil_window<3, TW>(tile, cy, cx, g, c, inv2s2, s, t, zero);
     stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
 }
 
@@ -165,15 +163,16 @@ __global__ __launch_bounds__(kTileThreads) void k_vnmap(const SessParams *__rest
 }
 
 // Copies the caller's device-resident frames into the sessions' raw-frame
-// buffers: one launch for a whole batch (blockIdx.y = session of the chunk),
-// 16-byte vector loads/stores where the frame size allows.
+// buffers (in each session's depth units, copied as 16-bit halves): one
+// launch for a whole batch (blockIdx.y = session of the chunk), 16-byte
+// vector loads/stores where the frame size allows.
 __global__ __launch_bounds__(256) void k_gather_depth(const SessParams *__restrict__ SP, DepthChunk ch) {
     const int s = blockIdx.y;
     if (s >= ch.n) return;
     const SessParams &P = SP[ch.slot[s]];
-    const size_t npx = (size_t)P.lev[0].W * P.lev[0].H;
+    const size_t npx = (size_t)P.lev[0].W * P.lev[0].H * (P.raw_f32 ? 2 : 1);
     const uint16_t *src = ch.src[s];
-    uint16_t *dst = const_cast<uint16_t *>(P.raw);
+    uint16_t *dst = (uint16_t *)const_cast<void *>(P.raw);
     const bool vec = ((npx & 7) == 0) && ((((uintptr_t)src) & 15) == 0);
     if (vec) {
         const uint4 *s4 = reinterpret_cast<const uint4 *>(src);

@@ -45,6 +45,10 @@ void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_typ
                       int sd_slices, hipStream_t s);
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
                     hipStream_t s);
+// a single session's model maps of `level` (npx pixels) into caller buffers
+// (each nullable): depth, vertex and normal (3 floats per pixel), image
+void launch_maps_out(const SessParams *P, int level, size_t npx, float *depth, float *vtx, float *nrm, uint8_t *img,
+                     hipStream_t s);
 // Several pyramid levels in one launch: segment k of the grid is level
 // level[k]'s work list (passed by value).
 struct LevelWork {

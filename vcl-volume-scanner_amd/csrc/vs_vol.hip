@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
     __syncthreads();
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
-    const float td = (x < L.W && y < L.H) ? depth_m(ldg(P.raw, (size_t)y * L.W + x)) : 0.0f;
+    const float td = (x < L.W && y < L.H) ? raw_depth(P, (size_t)y * L.W + x) : 0.0f;
     if (td > 0.0f) {
         const M34 Tc = ld_m34(st->T_cg_f);
         const float *Tcg = Tc.m;
@@ -177,7 +177,7 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     int x, y;
     if (!rint_ref(u, x) || !rint_ref(v, y)) return false;
     if (x < 0 || x > L.W - 1 || y < 0 || y > L.H - 1) return false;
-    const float dm = depth_m(ldg(P.raw, (size_t)y * L.W + x));
+    const float dm = raw_depth(P, (size_t)y * L.W + x);
     if (!(dm > 0.0f)) return false;
     const float r1x = 1.0f * ((float)x - L.K[2]) / L.K[0], r1y = 1.0f * ((float)y - L.K[3]) / L.K[1];
     const float lambda_inv = 1.0f / sqrtf(r1x * r1x + r1y * r1y + 1.0f);
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             int iu, iv;
             if (!roundf_ref(u, iu) || !roundf_ref(v, iv)) continue;
             if (iu < 0 || iu > L.W - 1 || iv < 0 || iv > L.H - 1) continue;
-            const float dm = depth_m(ldg(P.raw, (size_t)iv * L.W + iu));
+            const float dm = raw_depth(P, (size_t)iv * L.W + iu);
             if (!(dm > 0.0f)) continue;
             const float td = dm - p.z;
             if (td < -P.mu) continue;
@@ -488,6 +488,52 @@ struct Vol {
         }
         return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
     }
+    // ---- instrumented form only: the bytes the reference's own lookups
+    // request, tap by tap, in its order and with its early exits.
+    // d_gtvu (tsdf_renderer.cu:68-97) at integer voxel (x, y, z): nothing
+    // outside the grid, the weight of a voxel without weight, weight and
+    // value otherwise; true when the tap succeeds.
+    __device__ __forceinline__ bool ref_tap(int x, int y, int z, unsigned long long &bytes) const {
+        if (!in_grid(x, y, z)) return false;
+        const float w = ld_w(Wt, vox_index(P, x, y, z));
+        bytes += sizeof(TW);
+        if (!(w > 0.0f)) return false;
+        bytes += sizeof(TV);
+        return true;
+    }
+    // a nearest tap (ROUNDF of the position); cls = 0 outside the grid, 1 no
+    // weight, 2 valid
+    __device__ __forceinline__ int ref_nearest(f3 v, unsigned long long &bytes) const {
+        int x, y, z;
+        if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z) || !in_grid(x, y, z)) return 0;
+        return ref_tap(x, y, z, bytes) ? 2 : 1;
+    }
+    // d_gtvi (:100-153): 4 taps of the front plane, then 4 of the back plane,
+    // up to the first failing one
+    __device__ __forceinline__ void ref_trilinear(f3 v, unsigned long long &bytes) const {
+        if (!(fabsf(v.x) < 2147483520.0f && fabsf(v.y) < 2147483520.0f && fabsf(v.z) < 2147483520.0f)) return;
+        const int ox = (int)v.x, oy = (int)v.y, oz = (int)v.z;
+        for (int k = 0; k < 8; ++k)
+            if (!ref_tap(ox + (k & 1), oy + ((k >> 1) & 1), oz + (k >> 2), bytes)) return;
+    }
+    // d_csnt (:288-464): foremost 4, front 12, back 12, backmost 4 taps, up
+    // to the first failing one
+    __device__ __forceinline__ void ref_normal(f3 p, unsigned long long &bytes) const {
+        f3 v;
+        if (!world_to_voxel(P, p, v)) return;
+        const int ox = (int)v.x, oy = (int)v.y, oz = (int)v.z;
+        const int dx1[4] = {0, 1, 0, 1}, dy1[4] = {0, 0, 1, 1};
+        const int dx2[12] = {0, 1, -1, 0, 1, 2, -1, 0, 1, 2, 0, 1};
+        const int dy2[12] = {-1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2};
+        for (int i = 0; i < 4; ++i)
+            if (!ref_tap(ox + dx1[i], oy + dy1[i], oz - 1, bytes)) return;
+        for (int i = 0; i < 12; ++i)
+            if (!ref_tap(ox + dx2[i], oy + dy2[i], oz, bytes)) return;
+        for (int i = 0; i < 12; ++i)
+            if (!ref_tap(ox + dx2[i], oy + dy2[i], oz + 1, bytes)) return;
+        for (int i = 0; i < 4; ++i)
+            if (!ref_tap(ox + dx1[i], oy + dy1[i], oz + 2, bytes)) return;
+    }
     // d_gtvi: trilinear with toInt = truncation; any invalid tap -> false,
     // 1.0. The 2x2x2 stencil is range-checked as a whole and its 8 values and
     // weights are loaded together; a voxel of an empty brick has weight 0, so
@@ -565,9 +611,28 @@ __device__ __forceinline__ int steps_outside(f3 v, f3 d, const int *vdim) {
     return k >= 1.0f ? (int)k : 0;
 }
 
+// Tap statistics of the instrumented march (the reference's lookups: what
+// it would request, not what this kernel loads).
+struct RayStats {
+    unsigned n_near = 0, n_tri = 0, n_out = 0, n_w0 = 0;
+    unsigned long long b_near = 0, b_tri = 0;
+    template <class TV, class TW>
+    __device__ __forceinline__ void nearest(const Vol<TV, TW> &V, f3 v) {
+        ++n_near;
+        const int c = V.ref_nearest(v, b_near);
+        n_out += c == 0;
+        n_w0 += c == 1;
+    }
+    template <class TV, class TW>
+    __device__ __forceinline__ void trilinear(const Vol<TV, TW> &V, f3 v) {
+        ++n_tri;
+        V.ref_trilinear(v, b_tri);
+    }
+};
+
 template <class TV, class TW, bool STATS>
 __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C, const float *Tcg, float px,
-                                         float py, f3 &hit, unsigned &n_near, unsigned &n_tri) {
+                                         float py, f3 &hit, RayStats &rs) {
     const float dist_min = C.dist_cc - C.r_cube, dist_max = C.dist_cc + C.r_cube;
     const f3 pe = transform(Tcg, back_project(px, py, C.K, dist_max));
     const f3 rd = normalised(sub(pe, C.cam));
@@ -576,7 +641,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
     f3 vox;
     world_to_voxel(V.P, ps, vox);
     float tsdf;
-    if (STATS) ++n_near;
+    if (STATS) rs.nearest(V, vox);
     const bool valid = V.nearest_valid(vox, tsdf);
     bool coarse;
     if (!valid) coarse = true;
@@ -617,7 +682,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         vox = add(vox, scale(step, rd));
         total += step;
         if (total > C.total_max) return false;
-        if (STATS) ++n_near;
+        if (STATS) rs.nearest(V, vox);
         // A ray that has left the grid for good can only end as a miss
         // (every later tap reads 1.0 and the step stays positive); the
         // instrumented form keeps marching to count the reference's taps.
@@ -656,7 +721,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
             continue;
         }
         if (tsdf > -1.0f && tsdf < 1.0f) {
-            if (STATS) ++n_tri;
+            if (STATS) rs.trilinear(V, vox);
             V.trilinear(vox, tsdf);
             if (tsdf < 0.0f) {
                 if (coarse) return false;
@@ -673,7 +738,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
     (void)behind;
     float step = C.step_scale * tsdf;
     vox = add(vox, scale(step, rd));
-    if (STATS) ++n_tri;
+    if (STATS) rs.trilinear(V, vox);
     V.trilinear(vox, tsdf);
     step = C.step_scale * tsdf;
     vox = add(vox, scale(step, rd));
@@ -857,10 +922,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     C.total_max = P.total_max;
     C.max_iter = P.max_iter;
     const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint8_t *)s_dist};
-    unsigned n_near = 0, n_tri = 0;
+    RayStats rs;
     f3 p;
     const M34 Tc = ld_m34(st->T_cg_f);
-    const bool ok = cast_ray<TV, TW, STATS>(V, C, Tc.m, (float)x, (float)y, p, n_near, n_tri);
+    const bool ok = cast_ray<TV, TW, STATS>(V, C, Tc.m, (float)x, (float)y, p, rs);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
         stg3(P.mvtx[level], t, p.x, p.y, p.z);  // world hit; the shading makes it the camera-frame vertex
@@ -871,8 +936,12 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
     if (STATS) {
-        atomicAdd(stats + 0, (unsigned long long)n_near);
-        atomicAdd(stats + 1, (unsigned long long)n_tri);
+        atomicAdd(stats + 0, (unsigned long long)rs.n_near);
+        atomicAdd(stats + 1, (unsigned long long)rs.n_tri);
+        atomicAdd(stats + 3, (unsigned long long)rs.n_out);
+        atomicAdd(stats + 4, (unsigned long long)rs.n_w0);
+        atomicAdd(stats + 5, rs.b_near);
+        atomicAdd(stats + 6, rs.b_tri);
     }
 }
 
@@ -897,6 +966,11 @@ __global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams
     if (h.x == -100.0f) return;
     const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, nullptr};
     const f3 p = {h.x, h.y, h.z};
+    if (STATS) {
+        unsigned long long b = 0;
+        V.ref_normal(p, b);
+        atomicAdd(stats + 7, b);
+    }
     f3 n;
     if (surface_normal(V, p, n)) {
         const M34 Tg = ld_m34(st->T_gc_f);
@@ -1241,6 +1315,39 @@ static void carve_t(const SessParams *P, const uint8_t *valid, unsigned *carved,
 }
 void launch_carve(const SessParams *P, int tt, int wt, const uint8_t *valid, unsigned *carved, hipStream_t s) {
     VS_DISPATCH(tt, wt, carve_t, P, valid, carved, s);
+}
+
+// The session's model maps of `level` written into caller buffers (any may be
+// null), in the layout of rmi_Render_Maps_for_ICP's outputs
+// (z_yooji_cuda_tsdf_renderer.cu:1642-1656, g_rmi :1069-1209): depth = camera
+// z of the hit or 0, vertex and normal 3 floats per pixel (-100 for a miss),
+// normal image one byte per pixel. Grid-stride over the pixels.
+__global__ __launch_bounds__(256) void k_maps_out(const SessParams *__restrict__ SP, int level, float *depth,
+                                                  float *vtx, float *nrm, uint8_t *img) {
+    const SessParams &P = SP[0];
+    const size_t n = (size_t)P.lev[level].W * P.lev[level].H;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const MapXYZ v = ldg3(P.mvtx[level], i);
+        if (depth) stg(depth, i, v.x == -100.0f ? 0.0f : v.z);
+        if (vtx) {
+            stg(vtx, 3 * i, v.x);
+            stg(vtx, 3 * i + 1, v.y);
+            stg(vtx, 3 * i + 2, v.z);
+        }
+        if (nrm) {
+            const MapXYZ q = ldg3(P.mnrm[level], i);
+            stg(nrm, 3 * i, q.x);
+            stg(nrm, 3 * i + 1, q.y);
+            stg(nrm, 3 * i + 2, q.z);
+        }
+        if (img) stg(img, i, ldg(P.mimg[level], i));
+    }
+}
+void launch_maps_out(const SessParams *P, int level, size_t npx, float *depth, float *vtx, float *nrm, uint8_t *img,
+                     hipStream_t s) {
+    const size_t nb = (npx + 255) / 256;
+    const int g = (int)(nb < 1024 ? nb : 1024);
+    if (g > 0) hipLaunchKernelGGL(k_maps_out, dim3(g), dim3(256), 0, s, P, level, depth, vtx, nrm, img);
 }
 
 }  // namespace vs

@@ -15,10 +15,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvscan.so")
 SCENE_PATH = os.path.join(HERE, "libvscan_scene.so")
 
-VS_OK, VS_EINVAL, VS_ENOMEM, VS_EHIP, VS_TRACK_LOST, VS_MOTION_REJECTED, VS_BUDGET, VS_TOO_FEW_POINTS = range(8)
+(VS_OK, VS_EINVAL, VS_ENOMEM, VS_EHIP, VS_TRACK_LOST, VS_MOTION_REJECTED, VS_BUDGET, VS_TOO_FEW_POINTS,
+ VS_SINGULAR) = range(9)
 VS_TSDF_F32, VS_TSDF_F16 = 0, 1
 VS_WEIGHT_F32, VS_WEIGHT_U8 = 0, 1
 VS_INTEGRATE_DEPTH, VS_INTEGRATE_COLOR = 0, 1
+VS_DEPTH_MM_U16, VS_DEPTH_M_F32 = 0, 1
 VS_TUNE_OFF, VS_TUNE_ONLINE, VS_TUNE_CACHED = 0, 1, 2
 
 
@@ -33,7 +35,7 @@ class SessionDesc(C.Structure):
         ("voxel", C.c_float), ("mu", C.c_float), ("max_weight", C.c_float),
         ("tsdf_type", C.c_int), ("weight_type", C.c_int),
         ("bil_sigma_s", C.c_float), ("bil_sigma_r", C.c_float), ("bil_radius", C.c_int),
-        ("integrate_mode", C.c_int),
+        ("integrate_mode", C.c_int), ("depth_units", C.c_int), ("icp_rel_pivot", C.c_float),
     ]
 
 
@@ -101,7 +103,8 @@ def lib():
                      "vs_server_step", "vs_server_usage", "vs_set_color", "vs_get_texture",
                      "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
                      "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot", "vs_depth_reproject",
-                     "vs_depth_crop_to_volume", "vs_volume_carve", "vs_ctx_set_tuning", "vs_ctx_tuning"]:
+                     "vs_depth_crop_to_volume", "vs_volume_carve", "vs_ctx_set_tuning", "vs_ctx_tuning",
+                     "vs_bilateral_filter_depth", "vs_integrate_depth", "vs_render_maps"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_set_tuning.argtypes = [P, C.c_int, C.c_char_p]
         L.vs_ctx_tuning.argtypes = [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -110,6 +113,9 @@ def lib():
         L.vs_session_reset.argtypes = [P, P, P]
         L.vs_set_depth.argtypes = [P, P, C.c_int, P]
         L.vs_bilateral.argtypes = [P, P]
+        L.vs_bilateral_filter_depth.argtypes = [P, P, C.c_int, P, P]
+        L.vs_integrate_depth.argtypes = [P, P, P]
+        L.vs_render_maps.argtypes = [P, C.c_int, P, P, P, P, P]
         L.vs_pyramid.argtypes = [P, P]
         L.vs_vertex_normal_maps.argtypes = [P, C.c_int, P, P, P]
         L.vs_track.argtypes = [P, P, P]
@@ -254,7 +260,8 @@ WORKLOADS = {
 }
 
 
-def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None, integrate_mode=VS_INTEGRATE_DEPTH):
+def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None, integrate_mode=VS_INTEGRATE_DEPTH,
+              depth_units=VS_DEPTH_MM_U16):
     d = SessionDesc()
     W = W or w["W"]
     H = H or w["H"]
@@ -277,6 +284,8 @@ def make_desc(w, tsdf=None, weight=None, n=None, W=None, H=None, K=None, integra
     d.bil_radius = 2
     d.icp_dist2 = (0.055 * w["cube"]) ** 2
     d.integrate_mode = integrate_mode
+    d.depth_units = depth_units
+    d.icp_rel_pivot = 0.0
     return d
 
 
@@ -403,8 +412,11 @@ class Session:
         check(lib().vs_volume_download_color(self.h, _ptr(out)), "vs_volume_download_color")
         return out
 
+    def frame_dtype(self):
+        return np.float32 if self.desc.depth_units == VS_DEPTH_M_F32 else np.uint16
+
     def set_depth(self, depth):
-        depth = np.ascontiguousarray(depth, np.uint16)
+        depth = np.ascontiguousarray(depth, self.frame_dtype())
         check(lib().vs_set_depth(self.h, _ptr(depth), 1, None), "vs_set_depth")
 
     def bilateral(self):
@@ -429,7 +441,7 @@ class Session:
         if depth is None:
             check(lib().vs_fuse_frame(self.h, None, 1, None), "vs_fuse_frame")
             return
-        depth = np.ascontiguousarray(depth, np.uint16)
+        depth = np.ascontiguousarray(depth, self.frame_dtype())
         check(lib().vs_fuse_frame(self.h, _ptr(depth), 1, None), "vs_fuse_frame")
 
     def depth_reproject(self, src_ptr, w, h, K_src, T_dst_src, dst_ptr=None, is_float=False):
@@ -525,7 +537,9 @@ class Session:
         return a.value, b.value
 
     def raycast_counts(self, level):
-        c = np.zeros(3, np.uint64)
+        """vs_raycast_counts: [nearest taps, trilinear evaluations, hits, nearest taps outside the grid,
+        nearest taps without weight, nearest-tap bytes, trilinear bytes, normal bytes]."""
+        c = np.zeros(8, np.uint64)
         check(lib().vs_raycast_counts(self.h, level, _ptr(c)), "vs_raycast_counts")
         return c
 
