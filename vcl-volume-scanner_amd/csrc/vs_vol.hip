@@ -207,87 +207,172 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     return true;
 }
 
+// Four consecutive voxels of a brick (x, x+1, x+2, x+3 of one row, x a
+// multiple of 4): one 16-, 8- or 4-byte access.
+__device__ __forceinline__ void ld4(const float *p, size_t i, float o[4]) {
+    const vs_f4 v = *(const VS_GLOBAL vs_f4 *)(p + i);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+typedef unsigned int vs_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld4(const __half *p, size_t i, float o[4]) {
+    const vs_u2 v = *(const VS_GLOBAL vs_u2 *)((const uint16_t *)p + i);
+    o[0] = __half2float(__ushort_as_half((unsigned short)(v.x & 0xFFFFu)));
+    o[1] = __half2float(__ushort_as_half((unsigned short)(v.x >> 16)));
+    o[2] = __half2float(__ushort_as_half((unsigned short)(v.y & 0xFFFFu)));
+    o[3] = __half2float(__ushort_as_half((unsigned short)(v.y >> 16)));
+}
+__device__ __forceinline__ void ld4(const uint8_t *p, size_t i, float o[4]) {
+    const uint32_t v = *(const VS_GLOBAL uint32_t *)(p + i);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (float)((v >> (8 * k)) & 255u);
+}
+__device__ __forceinline__ void st4(float *p, size_t i, const float v[4]) {
+    *(VS_GLOBAL vs_f4 *)(p + i) = vs_f4{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void st4(__half *p, size_t i, const float v[4]) {
+    uint32_t h[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __half_as_ushort(__float2half_rn(v[k]));
+    *(VS_GLOBAL vs_u2 *)((uint16_t *)p + i) = vs_u2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+}
+__device__ __forceinline__ void st4(uint8_t *p, size_t i, const float v[4]) {
+    uint32_t w = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w |= (uint32_t)(uint8_t)v[k] << (8 * k);
+    *(VS_GLOBAL uint32_t *)(p + i) = w;
+}
+
+// The integrate walks the brick list in chunks of kIntegChunk consecutive
+// entries. The list comes from k_vis_mark in (session, image tile) order, so
+// a chunk's bricks mostly belong to one session and lie behind one small
+// patch of its depth frame. Workgroup g runs on XCD g % 8 (workgroups are
+// dealt to the eight XCDs in turn), and the workgroups of one XCD take the
+// chunks of one contiguous eighth of the list: the depth rows that
+// neighbouring bricks read are then fetched into that XCD's L2 once, not
+// into eight of them.
+constexpr int kIntegChunk = 8;
+constexpr int kXcds = 8;
+
+// One workgroup step takes two bricks, 128 threads each; a thread updates
+// four voxels of one brick row (x0..x0+3, y, z), so a wave covers 256
+// consecutive voxels (z-slices of 64). Voxel arithmetic per voxel as
+// g_ctv_Compute_TSDF_of_Voxels (volume_integrator.cu:154-262). Values and
+// weights are read and written in whole 128-voxel groups (32 threads): a
+// group with any updated voxel loads all of its voxels and stores them all
+// back, unchanged ones as they were, so its memory lines are written whole
+// (a partly written line costs the memory a read-modify-write); a group
+// without updates touches neither. Per-brick updated counts and band flags
+// go to LDS; after each chunk one barrier, then thread k < 8 sets brick k's
+// occupancy and band bits and thread 0 adds the chunk's counts.
 template <class TV, class TW>
 __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__restrict__ SP, BrickList bl) {
     const unsigned n = min(*bl.count, bl.capacity);
+    const unsigned nch = (n + kIntegChunk - 1) / kIntegChunk;
+    const unsigned G = gridDim.x;
+    const unsigned nx = (G % kXcds == 0) ? kXcds : 1u;
+    const unsigned xcd = blockIdx.x % nx, R = G / nx, r = blockIdx.x / nx;
+    const unsigned c_lo = (unsigned)((unsigned long long)nch * xcd / nx);
+    const unsigned c_hi = (unsigned)((unsigned long long)nch * (xcd + 1) / nx);
     const int t = threadIdx.x;
-    const int xl = t & 7, yl = (t >> 3) & 7, zl = t >> 6;
-    // per-wave counts of a brick, double-buffered by brick parity so that one
-    // barrier per brick suffices: a wave can be one brick ahead of thread 0,
-    // never two
-    __shared__ unsigned s_upd[2][kTileThreads / 64], s_band[2][kTileThreads / 64];
+    const int half = __builtin_amdgcn_readfirstlane(t >> 7);  // which brick of the pair (wave-uniform)
+    const int lt = t & 127, lane = t & 63, wib = (t >> 6) & 1;
+    const int xl = (lt & 1) * 4, yl = (lt >> 1) & 7, zl = lt >> 4;
+    const int li = zl * 64 + yl * 8 + xl;
+    // [chunk parity][brick of the chunk][wave of the brick]
+    __shared__ unsigned s_upd[2][kIntegChunk][2], s_band[2][kIntegChunk][2];
+    __shared__ int2 s_item[2][kIntegChunk];
     int parity = 0;
+    int cur = -1;  // session whose pose this wave holds
+    M34 Tg = {};
     // updated-voxel count of the current run of same-session bricks (thread 0)
     int run_sess = -1;
     unsigned long long run_upd = 0ull;
-    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
-        const int2 it = bl.items[i];
-        const SessParams &P = SP[it.x];
-        const SessState *st = P.st;
-        const int b = it.y;
-        const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
-        TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
-        TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
-        const LevelDesc &L = P.lev[0];
-        const M34 Tg = ld_m34(st->T_gc_f);
-        const float *T = Tg.m;
-        const float gx = ((float)(bx * kBrick + xl) + 0.5f) / P.vox_inv + P.origin[0];
-        const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
-        int nupd = 0;
-        bool band = false;  // an updated voxel's new value is not exactly 1.0
-        if (P.integ_mode == 1) {
-            const M34 Tc = ld_m34(st->T_cg_f);
-            uint32_t *col = P.vcol + (size_t)b * kBrickVox;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int zc = bz * kBrick + zl + 4 * h;
-                const float gz = ((float)zc + 0.5f) / P.vox_inv + P.origin[2];
-                nupd += integrate_voxel_color(P, T, Tc.m, {gx, gy, gz}, F, Wt, col, t + kTileThreads * h, band);
+    for (unsigned j = c_lo + r; j < c_hi; j += R) {
+        const unsigned b0 = j * kIntegChunk;
+        const int nb = (int)min((unsigned)kIntegChunk, n - b0);
+        for (int k = half; k < nb; k += 2) {
+            const int2 it = bl.items[b0 + k];
+            const SessParams &P = SP[it.x];
+            if (it.x != cur) {
+                cur = it.x;
+                Tg = ld_m34(P.st->T_gc_f);
             }
-        } else {
+            const float *T = Tg.m;
+            const int b = it.y;
+            const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
+            TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
+            TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
+            const LevelDesc &L = P.lev[0];
+            const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
+            const float gz = ((float)(bz * kBrick + zl) + 0.5f) / P.vox_inv + P.origin[2];
+            int nupd = 0;
+            bool band = false;  // an updated voxel's new value is not exactly 1.0
+            if (P.integ_mode == 1) {
+                const M34 Tc = ld_m34(P.st->T_cg_f);
+                uint32_t *col = P.vcol + (size_t)b * kBrickVox;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int zc = bz * kBrick + zl + 4 * h;
-            const float gz = ((float)zc + 0.5f) / P.vox_inv + P.origin[2];
-            const f3 p = transform(T, {gx, gy, gz});
-            float u, v;
-            project(p, L.K, u, v);
-            int iu, iv;
-            if (!roundf_ref(u, iu) || !roundf_ref(v, iv)) continue;
-            if (iu < 0 || iu > L.W - 1 || iv < 0 || iv > L.H - 1) continue;
-            const float dm = raw_depth(P, (size_t)iv * L.W + iu);
-            if (!(dm > 0.0f)) continue;
-            const float td = dm - p.z;
-            if (td < -P.mu) continue;
-            const int li = t + kTileThreads * h;
-            const float tsd = fminf(1.0f, __fdividef(td, P.mu));
-            const float w = ld_w(Wt, li), f = ld_val(F, li);
-            const float nw = w + 1.0f;
-            const float nv = __fdividef(fmaf(w, f, tsd), nw);
-            band |= nv != 1.0f;
-            st_val(F, li, nv);
-            st_w(Wt, li, fminf(nw, P.max_w));
-            ++nupd;
-        }
-        }
-        const unsigned long long w1 = __ballot(nupd >= 1), w2 = __ballot(nupd >= 2), wb = __ballot(band);
-        if ((t & 63) == 0) {
-            s_upd[parity][t >> 6] = (unsigned)(__popcll(w1) + __popcll(w2));
-            s_band[parity][t >> 6] = wb != 0ull;
+                for (int q = 0; q < 4; ++q) {
+                    const float gx = ((float)(bx * kBrick + xl + q) + 0.5f) / P.vox_inv + P.origin[0];
+                    nupd += integrate_voxel_color(P, T, Tc.m, {gx, gy, gz}, F, Wt, col, li + q, band);
+                }
+            } else {
+                float tsd[4];
+                bool up[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float gx = ((float)(bx * kBrick + xl + q) + 0.5f) / P.vox_inv + P.origin[0];
+                    const f3 p = transform(T, {gx, gy, gz});
+                    float u, v;
+                    project(p, L.K, u, v);
+                    int iu = 0, iv = 0;
+                    bool ok = roundf_ref(u, iu) && roundf_ref(v, iv);
+                    ok = ok && iu >= 0 && iu <= L.W - 1 && iv >= 0 && iv <= L.H - 1;
+                    const float dm = ok ? raw_depth(P, (size_t)iv * L.W + iu) : 0.0f;
+                    const float td = dm - p.z;
+                    up[q] = ok && dm > 0.0f && !(td < -P.mu);
+                    tsd[q] = fminf(1.0f, __fdividef(td, P.mu));
+                    nupd += up[q];
+                }
+                const unsigned long long any = __ballot(nupd > 0);
+                if ((any >> (lane & 32)) & 0xFFFFFFFFull) {
+                    float f[4], w[4];
+                    ld4(F, li, f);
+                    ld4(Wt, li, w);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (!up[q]) continue;
+                        const float nw = w[q] + 1.0f;
+                        const float nv = __fdividef(fmaf(w[q], f[q], tsd[q]), nw);
+                        band |= nv != 1.0f;
+                        f[q] = nv;
+                        w[q] = fminf(nw, P.max_w);
+                    }
+                    st4(F, li, f);
+                    st4(Wt, li, w);
+                }
+            }
+            const unsigned c1 = (unsigned)__popcll(__ballot(nupd >= 1)), c2 = (unsigned)__popcll(__ballot(nupd >= 2));
+            const unsigned c3 = (unsigned)__popcll(__ballot(nupd >= 3)), c4 = (unsigned)__popcll(__ballot(nupd >= 4));
+            const bool wband = __ballot(band) != 0ull;
+            if (lane == 0) {
+                s_upd[parity][k][wib] = c1 + c2 + c3 + c4;
+                s_band[parity][k][wib] = wband;
+                if (wib == 0) s_item[parity][k] = it;
+            }
         }
         __syncthreads();
-        if (t == 0) {
-            unsigned upd = 0, cb = 0;
-#pragma unroll
-            for (int w = 0; w < kTileThreads / 64; ++w) {
-                upd += s_upd[parity][w];
-                cb |= s_band[parity][w];
-            }
+        if (t < nb) {
+            const unsigned upd = s_upd[parity][t][0] + s_upd[parity][t][1];
             if (upd > 0) {
                 // occupancy and band bits are only ever set during this
                 // launch, and earlier launches' writes (clears included) are
                 // visible at its start: a plain (cached) read that shows a
                 // bit proves it set, and a stale 0 costs one redundant atomic
+                const int2 it = s_item[parity][t];
+                const SessParams &P = SP[it.x];
+                const int b = it.y;
+                const bool cb = s_band[parity][t][0] | s_band[parity][t][1];
+                const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
                 const uint32_t bb = 1u << (b & 31);
                 const int sb = super_of_brick(P, bx, by, bz);
                 const uint32_t sbb = 1u << (sb & 31);
@@ -296,9 +381,15 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
                 if (cb && !(wbb & bb)) atomicOr(P.bband + (b >> 5), bb);
                 if (cb && !(ws & sbb) && !(atomicOr(P.sband + (sb >> 5), sbb) & sbb))
                     stg(&P.st->sdist_dirty, 0, 1);  // a new band superbrick: rebuild the distance field
-                if (it.x != run_sess) {
+            }
+        }
+        if (t == 0) {
+            for (int k = 0; k < nb; ++k) {
+                const unsigned upd = s_upd[parity][k][0] + s_upd[parity][k][1];
+                const int sess = s_item[parity][k].x;
+                if (sess != run_sess) {
                     if (run_upd) atomicAdd(&SP[run_sess].st->upd_count, run_upd);
-                    run_sess = it.x;
+                    run_sess = sess;
                     run_upd = 0ull;
                 }
                 run_upd += (unsigned long long)upd;

@@ -11,7 +11,9 @@ are fused in order, one per step.
 
 Latency of a frame = completion of the step that fused it (HIP event, GPU
 clock, referenced to an event synchronised when the loop starts) minus its
-arrival time on the host clock.
+arrival time on the host clock. A session's frames start arriving when it is
+admitted; the time it waited for admission (a full table or budget) is
+reported separately, as are the host's time in the open, close and step calls.
 """
 import collections
 import dataclasses
@@ -150,6 +152,7 @@ class ServeLoop:
         live = {}          # slot -> [request, next frame to submit, frames completed]
         inflight = collections.deque()  # (event, [(slot, arrival)], t_submit)
         lat, sizes, refused, opened, closed = [], [], 0, 0, 0
+        admit_wait, t_open_call, t_close_call, t_step_call = [], [], [], []
         peak_open, peak_bytes, lost = 0, 0, 0
         ev0 = hiprt.Event()
         ev0.record(stream)
@@ -182,19 +185,25 @@ class ServeLoop:
             for slot in [s for s, v in live.items() if v[2] == v[0].n_frames]:
                 st = srv.session(slot).status()
                 lost += st[0] != 0
+                tc = time.perf_counter()
                 srv.close_slot(slot)
+                t_close_call.append(time.perf_counter() - tc)
                 del live[slot]
                 closed += 1
             # admissions (in arrival order; a refused session waits)
             while waiting and waiting[0].t_open <= now and now < duration:
                 r = waiting[0]
+                tc = time.perf_counter()
                 slot = srv.open(self.desc_for(r.vol_n), T_gc0=self.pose0(r.scene))
+                t_open_call.append(time.perf_counter() - tc)
                 if slot is None:
                     refused += 1
                     break
                 waiting.popleft()
                 # the client starts streaming once its session is admitted
-                r.arrivals = r.arrivals + max(clock() - r.t_open, 0.0)
+                wait = max(clock() - r.t_open, 0.0)
+                admit_wait.append(wait)
+                r.arrivals = r.arrivals + wait
                 live[slot] = [r, 0, 0]
                 opened += 1
             n_open, used = srv.usage()
@@ -213,7 +222,9 @@ class ServeLoop:
                 members = [(s, float(v[0].arrivals[v[1]])) for s, v in ready]
                 for _, v in ready:
                     v[1] += 1
+                tc = time.perf_counter()
                 srv.step(slots, ptrs, stream)
+                t_step_call.append(time.perf_counter() - tc)
                 ev = hiprt.Event()
                 ev.record(stream)
                 inflight.append((ev, members, now))
@@ -233,13 +244,23 @@ class ServeLoop:
         ev0.destroy()
         n_fused = len(lat)
         lat = np.array(lat) if lat else np.zeros(1)
+
+        def dist_ms(xs):
+            a = np.array(xs) * 1e3 if xs else np.zeros(1)
+            return {"p50": round(float(np.percentile(a, 50)), 3), "p99": round(float(np.percentile(a, 99)), 3),
+                    "max": round(float(a.max()), 3), "n": len(xs)}
+
         return {
             "duration_s": round(wall, 3),
             "fused_frames": n_fused,
             "fused_frames_per_s": round(n_fused / wall, 2),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                           "max": round(float(lat.max()) * 1e3, 3)},
+                           "max": round(float(lat.max()) * 1e3, 3),
+                           "frames_over_33ms": int(np.sum(lat > 0.033))},
+            "admission_wait_ms": dist_ms(admit_wait),
+            "host_call_ms": {"open": dist_ms(t_open_call), "close": dist_ms(t_close_call),
+                             "step": dist_ms(t_step_call)},
             "sessions_opened": opened, "sessions_closed": closed, "open_at_end": len(live),
             "admissions_refused": refused, "peak_open": peak_open, "peak_session_bytes": int(peak_bytes),
             "sessions_lost_tracking": int(lost),
