@@ -256,6 +256,20 @@ vs_status vs_volume_download_color(vs_session *s, uint8_t *rgb_planar);
  * optional W_l*H_l validity mask. Diagnostic and parity entry point. */
 vs_status vs_icp_system(vs_session *s, int level, const float T_21[16], double ATA[36], double ATb[6],
                         long long *valid, uint8_t *mask_host);
+/* vs_icp_system without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops): no counterpart
+ * in the reference. The tile partials live in a fixed device array of the
+ * library (8192 tiles of 32x8 pixels, 1.9 MB per GPU, held until the process
+ * ends); a level with more tiles (over 2,097,152 pixels) is refused with
+ * VS_EINVAL. Asynchronous on `stream` (NULL: the session's stream), ordered
+ * after earlier work on that stream. sums_dev: 29 doubles on the device, JtJ's
+ * upper triangle row by row (21), Jtr (6), the valid count and the sum of
+ * squared residuals, bit-identical to vs_icp_system's; mask_dev (nullable):
+ * W_l*H_l bytes on the device, 1 per valid correspondence. Every call on one
+ * GPU shares the array, so calls that overlap on a GPU (other streams, other
+ * threads, other contexts) race. */
+vs_status vs_icp_system_noalloc(vs_session *s, int level, const float T_21[16], double *sums_dev,
+                                uint8_t *mask_dev, vs_stream stream);
 /* Per-stage kernel times of the last vs_fuse_frame on this session (ms):
  * [bilateral, pyramid, track, integrate, raycast]. Valid after sync when
  * timing was enabled with vs_session_set_timing. */

@@ -29,6 +29,17 @@ void launch_fuse_ok(const SessParams *P, int nsess, hipStream_t s);
 
 using namespace vs;
 
+// Scratch of vs_icp_system_noalloc: the per-tile partial sums of one ICP
+// system (kIcpVals doubles per 32x8 tile), for levels of up to
+// kNoAllocMaxTiles tiles (2,097,152 pixels). A file-scope device array: each
+// GPU holds its own copy until the process ends, and every call on that GPU
+// shares it. Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (vs_ctx::scratch,
+// taken in vs_ctx_create); the array's name is not a device address on the
+// host.
+constexpr int kNoAllocMaxTiles = 8192;
+static __device__ __align__(256) double scratch[(size_t)kNoAllocMaxTiles * kIcpVals];
+
 namespace {
 
 thread_local std::string g_err;
@@ -260,6 +271,7 @@ struct Batch {
 
 struct vs_ctx {
     int device = 0;
+    double *scratch = nullptr;  // this device's copy of `scratch` (vs_icp_system_noalloc)
     std::map<std::vector<vs_session *>, std::unique_ptr<Batch>> batches;
     std::vector<vs_session *> sessions;
     std::vector<vs_server *> servers;
@@ -793,8 +805,12 @@ vs_status vs_ctx_create(int device, vs_ctx **out) {
     HIP_TRY(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(VS_EINVAL, "device %d of %d", device, n);
     HIP_TRY(hipSetDevice(device));
+    double *scr = nullptr;
+    HIP_TRY(hipGetSymbolAddress((void **)&scr, HIP_SYMBOL(scratch)));
+    if (!scr) return fail(VS_EHIP, "hipGetSymbolAddress(scratch) returned a null address");
     vs_ctx *c = new vs_ctx();
     c->device = device;
+    c->scratch = scr;
     const char *ng = std::getenv("VS_NO_GRAPH");
     c->use_graph = !(ng && ng[0] == '1');
     *out = c;
@@ -1626,6 +1642,21 @@ vs_status vs_icp_system(vs_session *s, int level, const float T_21[16], double A
         const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
         HIP_TRY(hipMemcpy(mask_host, mask, n, hipMemcpyDeviceToHost));
     }
+    return VS_OK;
+}
+
+vs_status vs_icp_system_noalloc(vs_session *s, int level, const float T_21[16], double *sums_dev,
+                                uint8_t *mask_dev, vs_stream stream) {
+    if (!s || !T_21 || !sums_dev) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    const int nt = s->L.tiles[level];
+    if (nt > kNoAllocMaxTiles)
+        return fail(VS_EINVAL, "level %d has %d tiles of 32x8 pixels; vs_icp_system_noalloc holds at most %d", level,
+                    nt, kNoAllocMaxTiles);
+    hipStream_t h = pick(s, stream);
+    launch_icp_system(s->self.dP, s->self.dWork[level], nt, level, T_21, s->ctx->scratch, mask_dev, h);
+    launch_icp_sum_tiles(s->ctx->scratch, nt, sums_dev, h);
+    HIP_TRY(hipGetLastError());
     return VS_OK;
 }
 

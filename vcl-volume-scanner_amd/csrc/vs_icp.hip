@@ -551,6 +551,19 @@ void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s) {
 void launch_pose_derived(const SessParams *P, hipStream_t s) {
     hipLaunchKernelGGL(k_pose_derived, dim3(1), dim3(64), 0, s, P);
 }
+// Sum of one ICP system's tile partials in tile order: thread k adds term k
+// of tiles 0, 1, ..., nt-1 in FP64, the order of vs_icp_system's host loop,
+// so both give the same bits.
+__global__ void k_icp_sum_tiles(const double *__restrict__ part, int nt, double *__restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= kIcpVals) return;
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += ldg(part, (size_t)t * kIcpVals + k);
+    stg(out, k, s);
+}
+void launch_icp_sum_tiles(const double *part, int nt, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_icp_sum_tiles, dim3(1), dim3(64), 0, s, part, nt, out);
+}
 void launch_icp_system(const SessParams *P, const WorkItem *work, int n, int level, const float *T21,
                        double *out_part, uint8_t *mask, hipStream_t s) {
     Mat16f m;

@@ -31,6 +31,8 @@ void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s);
 // one system, no solve: writes 29 doubles (+ optional mask) for diagnostics
 void launch_icp_system(const SessParams *P, const WorkItem *work, int n, int level, const float *T21,
                        double *out_part, uint8_t *mask, hipStream_t s);
+// the 29 sums of nt tile partials, in tile order, into out (one workgroup)
+void launch_icp_sum_tiles(const double *part, int nt, double *out, hipStream_t s);
 
 // volume stages (vs_vol.hip)
 struct BrickList {

@@ -104,7 +104,7 @@ def lib():
                      "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
                      "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot", "vs_depth_reproject",
                      "vs_depth_crop_to_volume", "vs_volume_carve", "vs_ctx_set_tuning", "vs_ctx_tuning",
-                     "vs_bilateral_filter_depth", "vs_integrate_depth", "vs_render_maps"]:
+                     "vs_bilateral_filter_depth", "vs_integrate_depth", "vs_render_maps", "vs_icp_system_noalloc"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_set_tuning.argtypes = [P, C.c_int, C.c_char_p]
         L.vs_ctx_tuning.argtypes = [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -133,6 +133,7 @@ def lib():
         L.vs_volume_download.argtypes = [P, P, P, P]
         L.vs_volume_upload.argtypes = [P, P, P]
         L.vs_icp_system.argtypes = [P, C.c_int, P, P, P, C.POINTER(C.c_longlong), P]
+        L.vs_icp_system_noalloc.argtypes = [P, C.c_int, P, P, P, P]
         L.vs_session_set_timing.argtypes = [P, C.c_int]
         L.vs_get_stage_ms.argtypes = [P, P]
         L.vs_get_integrate_counts.argtypes = [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
