@@ -163,6 +163,15 @@ vs_status vs_pyramid(vs_session *s, vs_stream stream);
  * z_yooji_image_processor.cpp:119-227). Not used by vs_fuse_frame, which
  * back-projects inline like the reference GPU tracker. */
 vs_status vs_vertex_normal_maps(vs_session *s, int level, float *vtx_dev, float *nrm_dev, vs_stream stream);
+/* A6-A7 in the reference CPU loop's in-place order (enm_Estimate_Normal_Map,
+ * z_yooji_image_processor.cpp:155-227): a grazing pixel's vertex becomes -100
+ * before the pixels after it in raster order are processed, so a pixel whose
+ * left or upper neighbour was invalidated gets no normal (and, if grazing
+ * itself, keeps its vertex). vs_vertex_normal_maps evaluates every pixel on
+ * the input map instead (order-independent); the two differ only next to
+ * grazing pixels. */
+vs_status vs_vertex_normal_maps_inplace(vs_session *s, int level, float *vtx_dev, float *nrm_dev,
+                                        vs_stream stream);
 /* A8-A12: projective point-to-plane ICP of the frame pyramid against the
  * model maps of the previous raycast; updates the resident pose.
  * Replaces LGKvPoseTracker::tp_Track_Pose (z_yooji_cuda_pose_tracker.cu:790-1042).
@@ -285,9 +294,10 @@ vs_status vs_ctx_stage_ms(vs_ctx *ctx, float ms[8]);
 
 /* Launch-shape autotuning (no counterpart in the reference, whose launches
  * have fixed shapes). Three launches of a step have interchangeable shapes:
- * the ray block of a wave in the raycast march and in its shading (8x8, 16x4
- * or 32x2 rays) and the workgroup count of the persistent integrate (1024,
- * 2048, 4096 or 8192); every shape gives bit-identical results. With
+ * the ray block of a wave in the raycast march (8x8, 16x4 or 32x2 rays), the
+ * ray block (32x2 or 8x8) and register budget (4 or 5 waves per SIMD) of
+ * its shading, and the workgroup count of the integrate (1024 to 32768);
+ * every shape gives bit-identical results. With
  * VS_TUNE_ONLINE, a batch (vs_fuse_batch, a server's slot table, a session's
  * own vs_fuse_frame) whose size class (largest frame, largest volume edge
  * and session count rounded up to powers of two, storage types) has no
@@ -303,7 +313,7 @@ typedef enum { VS_TUNE_OFF = 0, VS_TUNE_ONLINE = 1, VS_TUNE_CACHED = 2 } vs_tune
 vs_status vs_ctx_set_tuning(vs_ctx *ctx, int mode, const char *cache_path);
 /* The context's settled shapes, one size class per line:
  * "<class>|<march block>|<shade block>|<integrate workgroups>|<means>", where
- * <means> is "m8x8,m16x4,m32x2|s32x2,s8x8|i1024,i2048,i4096,i8192,i16384,i32768" (mean stage
+ * <means> is "m8x8,m16x4,m32x2|s32x2,s8x8,s32x2w5,s8x8w5|i1024,i2048,i4096,i8192,i16384,i32768" (mean stage
  * ms per candidate, -1 for one not timed) or "cached" (from the file). Writes
  * at most len bytes including the terminating zero; *needed (nullable) = the
  * full length + 1. */

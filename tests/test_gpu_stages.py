@@ -111,6 +111,49 @@ def test_vertex_normal_parity(sess, stream):
         assert np.abs(gn - rn)[ok].max() <= 1e-5
 
 
+@pytest.mark.parametrize("cfg,frame", [(0, 4), (1, 6)])
+def test_vertex_normal_inplace_parity(ctx, cfg, frame):
+    """The reference loop's in-place order against the oracle's in-place
+    form: masks bit-exact at every level, and different from the
+    order-independent form next to grazing pixels."""
+    import hiprt
+
+    print(f"case: in-place vertex/normal maps, scene {cfg} frame {frame}", flush=True)
+    st = vscan.SceneStream(cfg, seed=1)
+    w = W0 if cfg == 0 else vscan.WORKLOADS["plain_640x480_256"]
+    d = vscan.make_desc(w, K=st.K, n=64)
+    s = vscan.Session(ctx, d, T_gc0=st.pose(0))
+    try:
+        s.set_depth(st.frame(frame))
+        s.bilateral()
+        s.pyramid()
+        K = orc.K_levels(d)
+        differ = 0
+        for l in range(3):
+            h, wl = s.level_shape(l)
+            v = hiprt.DeviceBuffer(h * wl * 12)
+            n = hiprt.DeviceBuffer(h * wl * 12)
+            try:
+                vscan.check(vscan.lib().vs_vertex_normal_maps_inplace(s.h, l, v.ptr, n.ptr, None), "vnmap inplace")
+                s.sync()
+                gv = v.download(np.zeros((h, wl, 3), np.float32))
+                gn = n.download(np.zeros((h, wl, 3), np.float32))
+            finally:
+                v.free()
+                n.free()
+            rv, rn = orc.vnmap(s.filtered(l), K[l], inplace=True)
+            ov, on = orc.vnmap(s.filtered(l), K[l], inplace=False)
+            assert np.array_equal(gv[..., 0] == -100, rv[..., 0] == -100), f"vertex mask level {l}"
+            assert np.array_equal(gn[..., 0] == -100, rn[..., 0] == -100), f"normal mask level {l}"
+            ok = rn[..., 0] != -100
+            assert np.abs(gv - rv)[rv[..., 0] != -100].max() <= 1e-6
+            assert np.abs(gn - rn)[ok].max() <= 1e-5
+            differ += int(np.sum((rn[..., 0] == -100) != (on[..., 0] == -100)))
+        assert differ > 0, "the scene has no grazing pixel next to a valid normal"
+    finally:
+        s.close()
+
+
 def _oracle_volume(stream, desc, frames):
     """FP64 oracle TSDF built from ground-truth poses, returned as float32."""
     vol = orc.volume(desc)

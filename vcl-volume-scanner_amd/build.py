@@ -41,21 +41,34 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
+# Variant builds for measured comparisons (`python build.py --variant NAME`
+# writes libvscan_NAME.so next to libvscan.so; tests and the benchmark load
+# libvscan.so, or the file VSCAN_LIB names):
+#   icp_lds: the ICP iteration's block sums as per-thread FP32 FMAs and LDS
+#            chunk sums instead of on the matrix cores (VS_ICP_MFMA=0)
+VARIANTS = {"icp_lds": ["-DVS_ICP_MFMA=0"]}
+
+
+def build(verbose=False, variant=None):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     cmds = []
+    extra = VARIANTS[variant] if variant else []
+    tag = f"_{variant}" if variant else ""
     for src in HIP_SOURCES:
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        obj = os.path.join(BUILD, src.replace(".hip", f"{tag}.o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         for r in ex.map(_run, cmds):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout + r.stderr)
-    tmp = LIB + ".tmp"
+    lib = LIB.replace(".so", f"{tag}.so")
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+    if variant:
+        return lib, None
     tmp = SCENE_LIB + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", f"-I{INCLUDE}",
           os.path.join(CSRC, "scene.cpp"), "-o", tmp])
@@ -64,4 +77,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    var = sys.argv[sys.argv.index("--variant") + 1] if "--variant" in sys.argv else None
+    print(build(verbose="-v" in sys.argv, variant=var))

@@ -67,6 +67,7 @@ struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
     size_t tsdf, weight, bvis, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
+    size_t wtab;                          // voxel-centre world coordinates, x then y then z
     size_t total;
     size_t nvox;
     uint32_t nbricks, nwords, nsuper, nswords;
@@ -138,6 +139,7 @@ Layout make_layout(const vs_session_desc &d) {
     }
     L.stats = take(64);
     L.keys = take((size_t)W * H * 4);  // depth reprojection z-buffer
+    L.wtab = take((size_t)(d.vol_dim[0] + d.vol_dim[1] + d.vol_dim[2]) * 4);
     if (d.integrate_mode == VS_INTEGRATE_COLOR) {
         L.vcol = take(L.nvox * 4);
         L.rgb = take((size_t)W * H * 3);
@@ -213,19 +215,27 @@ struct Timer {
 // persistent integrate. Every shape gives the same results; which is fastest
 // depends on the workload, so a context can pick them per size class of
 // batch (frame size, volume size, storage types, session count).
+// The shading's candidates: ray block and the waves per SIMD its register
+// allocation aims for (4: every stencil load in flight at ~126 VGPRs; 5: 91
+// VGPRs).
+struct ShadeOpt {
+    int block, waves;
+    const char *name;
+};
+constexpr ShadeOpt kShadeOpts[] = {
+    {kRay32x2, 4, "32x2"}, {kRay8x8, 4, "8x8"}, {kRay32x2, 5, "32x2w5"}, {kRay8x8, 5, "8x8w5"}};
 struct Shape {
-    int march = kRay8x8, shade = kRay32x2;
+    int march = kRay8x8, shade = 0;  // shade: index into kShadeOpts
     int integ = -1;  // index into kIntegGrids; -1: the batch's default grid
 };
 constexpr int kMarchOpts[] = {kRay8x8, kRay16x4, kRay32x2};
-constexpr int kShadeOpts[] = {kRay32x2, kRay8x8};
 constexpr int kIntegGrids[] = {1024, 2048, 4096, 8192, 16384, 32768};
-constexpr int kNumMarch = 3, kNumShade = 2, kNumInteg = 6;  // kTuneSteps is a multiple of each
+constexpr int kNumMarch = 3, kNumShade = 4, kNumInteg = 6;  // kTuneSteps is a multiple of each
 const char *const kRayBlockName[] = {"8x8", "16x4", "32x2"};
 // Online tuning: the first kTuneSteps steps of a batch whose size class has
 // no choice yet run eagerly and timed, step i with march candidate i % 3,
-// shade candidate i % 2 and integrate grid i % 4 (each candidate of each
-// stage is timed 2 to 6 times, on consecutive frames); then each stage keeps
+// shade candidate i % 4 and integrate grid i % 6 (each candidate of each
+// stage is timed 2 to 4 times, on consecutive frames); then each stage keeps
 // the candidate with the lowest mean time and the step is captured with it.
 constexpr int kTuneSteps = 12;
 static_assert(kTuneSteps % kNumMarch == 0 && kTuneSteps % kNumShade == 0 && kTuneSteps % kNumInteg == 0,
@@ -556,7 +566,8 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     const LevelWork lw = level_work(B.dWork, B.nWork, B.levels);
     launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, s);
     mark(6);
-    launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, B.shape.shade, s);
+    launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, kShadeOpts[B.shape.shade].block,
+                         kShadeOpts[B.shape.shade].waves, s);
     mark(7);
     mark(8);
     if (timed) timed->used = true;
@@ -564,7 +575,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
 
 std::string shape_text(const Shape &h, const Batch &B) {
     char buf[96];
-    snprintf(buf, sizeof(buf), "march=%s shade=%s integrate=%d", kRayBlockName[h.march], kRayBlockName[h.shade],
+    snprintf(buf, sizeof(buf), "march=%s shade=%s integrate=%d", kRayBlockName[h.march], kShadeOpts[h.shade].name,
              h.integ < 0 ? B.integ_grid : kIntegGrids[h.integ]);
     return buf;
 }
@@ -575,9 +586,9 @@ void save_tuning(const vs_ctx *ctx) {
     if (ctx->tune_path.empty()) return;
     FILE *f = fopen(ctx->tune_path.c_str(), "w");
     if (!f) return;
-    fprintf(f, "# vscan launch shapes v1: size class|march ray block|shade ray block|integrate workgroups\n");
+    fprintf(f, "# vscan launch shapes v2: size class|march ray block|shade ray block[w waves]|integrate workgroups\n");
     for (const auto &kv : ctx->tuned)
-        fprintf(f, "%s|%s|%s|%d\n", kv.first.c_str(), kRayBlockName[kv.second.march], kRayBlockName[kv.second.shade],
+        fprintf(f, "%s|%s|%s|%d\n", kv.first.c_str(), kRayBlockName[kv.second.march], kShadeOpts[kv.second.shade].name,
                 kIntegGrids[kv.second.integ]);
     fclose(f);
 }
@@ -600,7 +611,9 @@ void load_tuning(vs_ctx *ctx) {
         if (nf != 4) continue;
         Shape h;
         h.march = block_index(fld[1]);
-        h.shade = block_index(fld[2]);
+        h.shade = -1;
+        for (int i = 0; i < kNumShade; ++i)
+            if (!strcmp(fld[2], kShadeOpts[i].name)) h.shade = i;
         const int g = atoi(fld[3]);
         for (int i = 0; i < kNumInteg; ++i)
             if (kIntegGrids[i] == g) h.integ = i;
@@ -633,7 +646,7 @@ vs_status tune_step(vs_ctx *ctx, Batch &B, hipStream_t s) {
         T.ci[1] = T.step % kNumShade;
         T.ci[2] = T.step % kNumInteg;
         B.shape.march = kMarchOpts[T.ci[0]];
-        B.shape.shade = kShadeOpts[T.ci[1]];
+        B.shape.shade = T.ci[1];
         B.shape.integ = T.ci[2];
         HIP_TRY(T.timer.init());
         enqueue_step(B, s, &T.timer);
@@ -659,7 +672,7 @@ vs_status tune_step(vs_ctx *ctx, Batch &B, hipStream_t s) {
         }
     }
     B.shape.march = kMarchOpts[best[0]];
-    B.shape.shade = kShadeOpts[best[1]];
+    B.shape.shade = best[1];
     B.shape.integ = best[2];
     B.tuned = true;
     B.tune.reset();
@@ -928,6 +941,22 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     for (int k = 0; k < 3; ++k) P.sdim[k] = L.sdim[k];
     P.sband_words = (int)L.nswords;
     P.icp_part = (double *)(m + L.icp_part);
+    {
+        float *wt = (float *)(m + L.wtab);
+        P.wtab[0] = wt;
+        P.wtab[1] = wt + d.vol_dim[0];
+        P.wtab[2] = wt + d.vol_dim[0] + d.vol_dim[1];
+        // the same FP32 operations as the device would do (division, then
+        // addition, each rounded once; this file is built without contraction)
+        std::vector<float> h((size_t)d.vol_dim[0] + d.vol_dim[1] + d.vol_dim[2]);
+        size_t o = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int i = 0; i < d.vol_dim[a]; ++i) {
+                volatile float q = ((float)i + 0.5f) / P.vox_inv;
+                h[o++] = q + P.origin[a];
+            }
+        HIP_TRY(hipMemcpy(wt, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
     P.st = (SessState *)(m + L.state);
     P.tsdf_type = d.tsdf_type;
     P.weight_type = d.weight_type;
@@ -1102,6 +1131,17 @@ vs_status vs_vertex_normal_maps(vs_session *s, int level, float *vtx_dev, float 
     if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
     hipStream_t h = pick(s, stream);
     launch_vnmap(s->self.dP, s->self.dWork[level], s->self.nWork[level], level, vtx_dev, nrm_dev, h);
+    HIP_TRY(hipGetLastError());
+    return VS_OK;
+}
+
+vs_status vs_vertex_normal_maps_inplace(vs_session *s, int level, float *vtx_dev, float *nrm_dev,
+                                        vs_stream stream) {
+    if (!s || !vtx_dev || !nrm_dev) return fail(VS_EINVAL, "null argument");
+    if (level < 0 || level >= s->d.levels) return fail(VS_EINVAL, "level %d", level);
+    hipStream_t h = pick(s, stream);
+    launch_vnmap(s->self.dP, s->self.dWork[level], s->self.nWork[level], level, vtx_dev, nrm_dev, h);
+    launch_vnmap_inplace(s->self.dP, level, vtx_dev, nrm_dev, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
 }
@@ -1698,7 +1738,7 @@ vs_status vs_ctx_tuning(vs_ctx *ctx, char *buf, size_t len, size_t *needed) {
         const Shape &h = kv.second;
         char line[160];
         snprintf(line, sizeof(line), "%s|%s|%s|%d|", kv.first.c_str(), kRayBlockName[h.march],
-                 kRayBlockName[h.shade], kIntegGrids[h.integ]);
+                 kShadeOpts[h.shade].name, kIntegGrids[h.integ]);
         out += line;
         auto m = ctx->tune_ms.find(kv.first);
         out += m != ctx->tune_ms.end() ? m->second : std::string("cached");

@@ -125,6 +125,10 @@ struct alignas(16) SessParams {
     int sdim[3];             // superbricks per axis
     int sband_words;         // 32-bit words of sband
     double *icp_part;        // ICP group partials (kIcpVals per group of level-0 tiles)
+    // World coordinate of each voxel centre along each axis, the reference's
+    // FP32 expression ((float)i + 0.5f) / vox_inv + origin[a] (d_gpw,
+    // _yooji_cuda_math.h:285-302), evaluated once per session: wtab[a][i]
+    const float *wtab[3];
     SessState *st;
     int tsdf_type, weight_type;
     int sid;                 // index in the context's table

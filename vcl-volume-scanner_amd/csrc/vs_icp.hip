@@ -248,6 +248,49 @@ __device__ __forceinline__ void block_terms(float v[kIcpVals], double *red64) {
     __syncthreads();
 }
 
+// VS_ICP_MFMA (1 in the default build; build.py --variant icp_lds builds the
+// per-thread FMA sums and LDS chunk sums instead): the block's 29 sums from
+// the matrix cores. For each tile of the group, each pixel's row
+// x = [A (6), b, ok] goes to LDS (2 KB per wave) and the wave runs
+// v_mfma_f32_16x16x4f32 over its 64 rows, 8 pixels per instruction:
+// the 16x16 product packs two 8x8 Gram blocks (two sets of 4 pixels) on its
+// diagonal, with x read as both operands (lane l supplies feature l % 8 of
+// pixel 4 * ((l % 16) / 8) + l / 16 of the instruction's 8), so
+// G = sum x x^T holds J^T J, J^T r, sum r^2 and the count. The two diagonal
+// blocks are added across lanes (lane l and l + 40 hold matching entries),
+// and the 4 waves' blocks are added in FP64 in wave order. Without the 29
+// per-thread accumulators and the 30 KB of LDS rows the kernel holds 64
+// registers and 9 KB of LDS, so 8 waves per SIMD are resident instead of 5:
+// the iteration waits on its dependent gathers (depth, then the model taps),
+// and the extra waves hide them (ICP stage 2.61 -> 2.22 ms per 64-session
+// step, with the same VALU work and bytes fetched).
+#ifndef VS_ICP_MFMA
+#define VS_ICP_MFMA 1
+#endif
+#if VS_ICP_MFMA
+typedef float vs_f4x __attribute__((ext_vector_type(4)));
+// entry (i, j) of the 8x8 Gram block that holds term k of the 29
+__device__ __forceinline__ void term_ij(int k, int &i, int &j) {
+    if (k < 21) {
+        int r = 0, n = 6;
+        while (k >= n) {
+            k -= n;
+            ++r;
+            --n;
+        }
+        i = r;
+        j = r + k;
+    } else if (k < 27) {
+        i = k - 21;
+        j = 6;
+    } else if (k == 27) {
+        i = j = 7;
+    } else {
+        i = j = 6;
+    }
+}
+#endif
+
 __device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *st, int level, int iter) {
     return !ldg(&st->do_track, 0) || ldg(&st->icp_failed, 0) || ldg(&st->conv_level, 0) == level ||
            iter >= P.iters[level] || (ldg(&st->force_single, 0) && level != 0);
@@ -255,9 +298,10 @@ __device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *s
 
 // One Gauss-Newton iteration at `level`, first half: the 29 sums of every
 // group of kIcpGroup consecutive 32x8 tiles of every session of the work
-// list (items are (session, group)), one FP64 partial per group. Each thread
-// accumulates its pixel of each tile of the group in FP32, then one block
-// reduction runs per group. No cross-workgroup synchronisation here: on a
+// list (items are (session, group)), one FP64 partial per group, summed on
+// the matrix cores (above) or, with VS_ICP_MFMA=0, by per-thread FP32
+// accumulation over the group's tiles and one block reduction (block_terms).
+// No cross-workgroup synchronisation here: on a
 // multi-XCD device an agent-scope release per workgroup would write back L2
 // each time, so the reduction and the solve are the next launch
 // (k_icp_solve).
@@ -275,9 +319,11 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     if (t0 >= ntiles) return;
     if (threadIdx.x < 16) T21s[threadIdx.x] = ldg(st->T_21_f, threadIdx.x);
     __syncthreads();
+#if !VS_ICP_MFMA
     float acc[kIcpVals];
 #pragma unroll
     for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0f;
+#endif
     // the group's depths first (independent loads; a pixel outside the
     // frame re-reads pixel 0 and is skipped), then the pixels in order
     int xs[kIcpGroup], ys[kIcpGroup];
@@ -291,6 +337,48 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
         in[g] = tile < ntiles && xs[g] < L.W && ys[g] < L.H;
         td[g] = ldg(P.filt[level], in[g] ? (size_t)ys[g] * L.W + xs[g] : (size_t)0);
     }
+#if VS_ICP_MFMA
+    __shared__ float feat[kTileThreads / 64][64][8];
+    __shared__ float gw[kTileThreads / 64][8][8];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int px = ((lane & 15) >> 3) * 4 + (lane >> 4), fi = lane & 7;
+    vs_f4x g4 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int g = 0; g < kIcpGroup; ++g) {
+        float A[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b = 0.0f;
+        bool ok = false;
+        if (in[g]) ok = icp_summand_d(P, level, xs[g], ys[g], td[g], T21s, A, b);
+        // the wave's 64 rows of this tile; a wave reads only its own rows
+        if (g > 0) __syncthreads();
+        float *f = feat[wv][lane];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) f[k] = A[k];
+        f[6] = b;
+        f[7] = ok ? 1.0f : 0.0f;
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const float a = feat[wv][8 * m + px][fi];
+            g4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, a, g4, 0, 0, 0);
+        }
+    }
+    // lane l holds D[4 (l / 16) + r][l % 16]; the second block's entry
+    // D[i + 8][j + 8] of the first block's D[i][j] sits in lane l + 40
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float other = __shfl_down(g4[r], 40, 64);
+        if ((lane >> 4) < 2 && (lane & 15) < 8) gw[wv][4 * (lane >> 4) + r][lane & 15] = g4[r] + other;
+    }
+    __syncthreads();
+    if (threadIdx.x < kIcpVals) {
+        int i, j;
+        term_ij(threadIdx.x, i, j);
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kTileThreads / 64; ++w) s += (double)gw[w][i][j];
+        stg(P.icp_part, (size_t)wi.tile * kIcpVals + threadIdx.x, s);
+    }
+#else
 #pragma unroll
     for (int g = 0; g < kIcpGroup; ++g) {
         if (in[g]) {
@@ -301,6 +389,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     }
     block_terms(acc, red64);
     if (threadIdx.x < kIcpVals) stg(P.icp_part, (size_t)wi.tile * kIcpVals + threadIdx.x, red64[threadIdx.x]);
+#endif
 }
 
 // Second half: one workgroup per session sums the group partials in FP64 in

@@ -162,6 +162,90 @@ __global__ __launch_bounds__(kTileThreads) void k_vnmap(const SessParams *__rest
     nrm[3 * t + 0] = o[3]; nrm[3 * t + 1] = o[4]; nrm[3 * t + 2] = o[5];
 }
 
+// The reference CPU loop's in-place order (enm_Estimate_Normal_Map,
+// z_yooji_image_processor.cpp:155-227, which writes the grazing invalidation
+// into the vertex map it is still reading): pixels after a grazing pixel in
+// raster order see its vertex as -100. A grazing pixel is invalidated only
+// when neither its left nor its upper neighbour was:
+//     inv(i, j) = c(i, j) & !inv(i, j-1) & !inv(i-1, j)
+// where c marks the grazing pixels of the order-independent pass (k_vnmap,
+// which has written vtx/nrm already). A grazing pixel that is not
+// invalidated keeps its vertex (its normal was never computed), and a pixel
+// whose left or upper neighbour is invalidated loses its normal. One wave
+// walks the rows: it reads a row's flags as 64-bit ballots, resolves the
+// recurrence along the row word by word (runs of c alternate, starting
+// invalidated unless the pixel before was), and applies the fix-ups.
+__global__ __launch_bounds__(64) void k_vnmap_inplace(const SessParams *__restrict__ SP, int l, float *vtx,
+                                                      float *nrm) {
+    const SessParams &P = SP[0];
+    const LevelDesc &L = P.lev[l];
+    const int W = L.W, H = L.H, lane = threadIdx.x;
+    const int nw = (W + 63) >> 6;  // <= 128 words (frames are at most 8192 wide)
+    __shared__ unsigned long long s_up[128], s_c[128], s_n[128], s_inv[128];
+    for (int k = lane; k < nw; k += 64) s_up[k] = 0ull;
+    const float *D = P.filt[l];
+    const float *K = L.K;
+    __syncthreads();
+    for (int i = 1; i < H - 1; ++i) {
+        for (int k = 0; k < nw; ++k) {
+            const int j = (k << 6) + lane;
+            const bool in = j >= 1 && j < W - 1;
+            const size_t t = (size_t)i * W + j;
+            const float vx = in ? vtx[3 * t] : 0.0f;
+            const float nx = in ? nrm[3 * t] : -100.0f;
+            const float d = in ? ldg(D, t) : 0.0f;
+            const unsigned long long cm = __ballot(in && vx == -100.0f && d > 0.0f);
+            const unsigned long long nm = __ballot(in && nx != -100.0f);
+            if (lane == 0) {
+                s_c[k] = cm;
+                s_n[k] = nm;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            bool carry = false;  // inv of the pixel before the word
+            for (int k = 0; k < nw; ++k) {
+                unsigned long long rest = s_c[k] & ~s_up[k], inv = 0ull;
+                while (rest) {
+                    const int s0 = __builtin_ctzll(rest);
+                    const unsigned long long from = rest >> s0;
+                    const int len = (~from == 0ull) ? 64 - s0 : __builtin_ctzll(~from);
+                    const unsigned long long run = (len >= 64 ? ~0ull : ((1ull << len) - 1ull)) << s0;
+                    const int first = s0 + ((s0 == 0 && carry) ? 1 : 0);
+                    inv |= run & ((first & 1) ? 0xAAAAAAAAAAAAAAAAull : 0x5555555555555555ull);
+                    rest &= ~run;
+                }
+                s_inv[k] = inv;
+                carry = (inv >> 63) & 1ull;
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < nw; ++k) {
+            const int j = (k << 6) + lane;
+            if (!(j >= 1 && j < W - 1)) continue;
+            const unsigned long long inv = s_inv[k];
+            const bool c = (s_c[k] >> lane) & 1ull, nv = (s_n[k] >> lane) & 1ull;
+            const bool here = (inv >> lane) & 1ull;
+            const bool left = lane ? ((inv >> (lane - 1)) & 1ull) : (k ? ((s_inv[k - 1] >> 63) & 1ull) : false);
+            const bool up = (s_up[k] >> lane) & 1ull;
+            const size_t t = (size_t)i * W + j;
+            if (c && !here) {
+                const float d = ldg(D, t);
+                vtx[3 * t + 0] = d * ((float)j - K[2]) / K[0];
+                vtx[3 * t + 1] = d * ((float)i - K[3]) / K[1];
+                vtx[3 * t + 2] = d;
+            } else if (nv && (left || up)) {
+                nrm[3 * t + 0] = -100.0f;
+                nrm[3 * t + 1] = -100.0f;
+                nrm[3 * t + 2] = -100.0f;
+            }
+        }
+        __syncthreads();
+        for (int k = lane; k < nw; k += 64) s_up[k] = s_inv[k];
+        __syncthreads();
+    }
+}
+
 // Copies the caller's device-resident frames into the sessions' raw-frame
 // buffers (in each session's depth units, copied as 16-bit halves): one
 // launch for a whole batch (blockIdx.y = session of the chunk), 16-byte
@@ -197,6 +281,9 @@ void launch_pyr_down(const SessParams *P, const WorkItem *work, int n, int l, hi
 }
 void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int l, float *vtx, float *nrm, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_vnmap, dim3(n), dim3(kTileThreads), 0, s, P, work, l, vtx, nrm);
+}
+void launch_vnmap_inplace(const SessParams *P, int l, float *vtx, float *nrm, hipStream_t s) {
+    hipLaunchKernelGGL(k_vnmap_inplace, dim3(1), dim3(64), 0, s, P, l, vtx, nrm);
 }
 
 }  // namespace vs

@@ -18,6 +18,8 @@ void launch_gather_depth(const SessParams *P, const DepthChunk &ch, hipStream_t 
 void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s);
 void launch_pyr_down(const SessParams *P, const WorkItem *work, int n, int level, hipStream_t s);
 void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int level, float *vtx, float *nrm, hipStream_t s);
+// single session (P[0]): after launch_vnmap, the reference loop's in-place order
+void launch_vnmap_inplace(const SessParams *P, int level, float *vtx, float *nrm, hipStream_t s);
 
 // tracker (vs_icp.hip); nsess = sessions in the batch
 void launch_track_setup(const SessParams *P, int nsess, const double *T_init, hipStream_t s);
@@ -72,8 +74,9 @@ void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipS
 enum RayBlock { kRay8x8 = 0, kRay16x4 = 1, kRay32x2 = 2 };
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
                           unsigned long long *stats, int ray_block, hipStream_t s);
+// waves: the waves per SIMD its registers aim for (4, 5 or 6)
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
-                          unsigned long long *stats, int ray_block, hipStream_t s);
+                          unsigned long long *stats, int ray_block, int waves, hipStream_t s);
 // volume maintenance (single session, P points at its parameters)
 void launch_volume_init(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, uint32_t nwords,
                         int sd_slices, hipStream_t s);

@@ -303,25 +303,30 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
             TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
             const LevelDesc &L = P.lev[0];
-            const float gy = ((float)(by * kBrick + yl) + 0.5f) / P.vox_inv + P.origin[1];
-            const float gz = ((float)(bz * kBrick + zl) + 0.5f) / P.vox_inv + P.origin[2];
+            // voxel-centre world coordinates from the session's tables
+            const float gy = ldg(P.wtab[1], by * kBrick + yl), gz = ldg(P.wtab[2], bz * kBrick + zl);
+            float gxs[4];
+            {
+                const vs_f4 g4 = *(const VS_GLOBAL vs_f4 *)(P.wtab[0] + bx * kBrick + xl);
+                gxs[0] = g4.x;
+                gxs[1] = g4.y;
+                gxs[2] = g4.z;
+                gxs[3] = g4.w;
+            }
             int nupd = 0;
             bool band = false;  // an updated voxel's new value is not exactly 1.0
             if (P.integ_mode == 1) {
                 const M34 Tc = ld_m34(P.st->T_cg_f);
                 uint32_t *col = P.vcol + (size_t)b * kBrickVox;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float gx = ((float)(bx * kBrick + xl + q) + 0.5f) / P.vox_inv + P.origin[0];
-                    nupd += integrate_voxel_color(P, T, Tc.m, {gx, gy, gz}, F, Wt, col, li + q, band);
-                }
+                for (int q = 0; q < 4; ++q)
+                    nupd += integrate_voxel_color(P, T, Tc.m, {gxs[q], gy, gz}, F, Wt, col, li + q, band);
             } else {
                 float tsd[4];
                 bool up[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const float gx = ((float)(bx * kBrick + xl + q) + 0.5f) / P.vox_inv + P.origin[0];
-                    const f3 p = transform(T, {gx, gy, gz});
+                    const f3 p = transform(T, {gxs[q], gy, gz});
                     float u, v;
                     project(p, L.K, u, v);
                     int iu = 0, iv = 0;
@@ -1036,9 +1041,14 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     }
 }
 
-template <class TV, class TW, bool STATS, int RB>
-__global__ __launch_bounds__(kTileThreads) void k_raycast_shade(const SessParams *__restrict__ SP, LevelWork lw,
-                                                                unsigned long long *stats) {
+// WV: the waves per SIMD the register allocation aims for (4: the compiler's
+// own choice, ~126 VGPRs with all 64 stencil loads in flight; 5: 91, fewer
+// loads in flight); the same results for each, so the autotuner picks. (6
+// waves, 80 VGPRs, spills a few registers and was slower still.)
+template <class TV, class TW, bool STATS, int RB, int WV>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(WV == 4 ? 1 : WV, WV == 4 ? 8 : WV)))
+void k_raycast_shade(
+    const SessParams *__restrict__ SP, LevelWork lw, unsigned long long *stats) {
     int level;
     const WorkItem wi = level_item(lw, level);
     const SessParams &P = SP[wi.sess];
@@ -1306,18 +1316,24 @@ static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long
         hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
 }
 template <class TV, class TW>
-static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, hipStream_t s) {
+static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, int waves,
+                    hipStream_t s) {
     const int n = lw.blocks();
     if (n <= 0) return;
+#define VS_SHADE(RB_, WV_) \
+    hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, RB_, WV_>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats)
     if (stats)
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true, kRay32x2>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true, kRay32x2, 4>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
                            stats);
+    else if (rb == kRay8x8 && waves == 5)
+        VS_SHADE(kRay8x8, 5);
     else if (rb == kRay8x8)
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
-                           stats);
+        VS_SHADE(kRay8x8, 4);
+    else if (waves == 5)
+        VS_SHADE(kRay32x2, 5);
     else
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, kRay32x2>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
-                           stats);
+        VS_SHADE(kRay32x2, 4);
+#undef VS_SHADE
 }
 LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLevels], int levels) {
     LevelWork lw{};
@@ -1335,8 +1351,8 @@ void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tt, int 
     VS_DISPATCH(tt, wt, march_t, P, lw, stats, rb, s);
 }
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
-                          int rb, hipStream_t s) {
-    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, rb, s);
+                          int rb, int waves, hipStream_t s) {
+    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, rb, waves, s);
 }
 static LevelWork one_level(const WorkItem *work, int n, int level) {
     LevelWork lw{};
@@ -1351,13 +1367,13 @@ static LevelWork one_level(const WorkItem *work, int n, int level) {
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
     launch_raycast_march(P, lw, tt, wt, nullptr, kRay8x8, s);
-    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay32x2, s);
+    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay32x2, 4, s);
 }
 void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
                           unsigned long long *stats, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
     launch_raycast_march(P, lw, tt, wt, stats, kRay8x8, s);
-    launch_raycast_shade(P, lw, tt, wt, stats, kRay32x2, s);
+    launch_raycast_shade(P, lw, tt, wt, stats, kRay32x2, 4, s);
 }
 
 static int grid_for(size_t n) {

@@ -135,8 +135,13 @@ class ServeLoop:
     of frame 0, `desc_for(vol_n)` the session descriptor. `max_batch` caps the
     frames per step (1 = one frame at a time, oldest arrival first)."""
 
-    def __init__(self, backend, desc_for, frame_ptr, pool_len, pose0, max_inflight=2, max_batch=None):
+    def __init__(self, backend, desc_for, frame_ptr, pool_len, pose0, max_inflight=2, max_batch=None,
+                 admit_per_pass=2):
         self.backend = backend
+        # sessions admitted per pass of the loop: opening one takes ~1-3 ms of
+        # host time (allocation, volume clear), so admitting a whole burst
+        # before stepping would hold back the frames of the open sessions
+        self.admit_per_pass = admit_per_pass
         self.desc_for, self.frame_ptr, self.pool_len, self.pose0 = desc_for, frame_ptr, pool_len, pose0
         self.max_inflight = max_inflight
         self.max_batch = max_batch
@@ -190,8 +195,10 @@ class ServeLoop:
                 t_close_call.append(time.perf_counter() - tc)
                 del live[slot]
                 closed += 1
-            # admissions (in arrival order; a refused session waits)
-            while waiting and waiting[0].t_open <= now and now < duration:
+            # admissions (in arrival order; a refused session waits), a few
+            # per pass so the open sessions' frames keep being stepped
+            admitted_now = 0
+            while waiting and waiting[0].t_open <= now and now < duration and admitted_now < self.admit_per_pass:
                 r = waiting[0]
                 tc = time.perf_counter()
                 slot = srv.open(self.desc_for(r.vol_n), T_gc0=self.pose0(r.scene))
@@ -206,6 +213,7 @@ class ServeLoop:
                 r.arrivals = r.arrivals + wait
                 live[slot] = [r, 0, 0]
                 opened += 1
+                admitted_now += 1
             n_open, used = srv.usage()
             peak_open, peak_bytes = max(peak_open, n_open), max(peak_bytes, used)
             now = clock()

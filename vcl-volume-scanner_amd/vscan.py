@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvscan.so")
+# VSCAN_LIB may name another build of the library (build.py --variant)
+LIB_PATH = os.environ.get("VSCAN_LIB") or os.path.join(HERE, "libvscan.so")
 SCENE_PATH = os.path.join(HERE, "libvscan_scene.so")
 
 (VS_OK, VS_EINVAL, VS_ENOMEM, VS_EHIP, VS_TRACK_LOST, VS_MOTION_REJECTED, VS_BUDGET, VS_TOO_FEW_POINTS,
@@ -104,7 +105,8 @@ def lib():
                      "vs_volume_download_color", "vs_session_snapshot_bytes", "vs_session_snapshot",
                      "vs_session_restore", "vs_server_evict", "vs_server_open_snapshot", "vs_depth_reproject",
                      "vs_depth_crop_to_volume", "vs_volume_carve", "vs_ctx_set_tuning", "vs_ctx_tuning",
-                     "vs_bilateral_filter_depth", "vs_integrate_depth", "vs_render_maps", "vs_icp_system_noalloc"]:
+                     "vs_bilateral_filter_depth", "vs_integrate_depth", "vs_render_maps", "vs_icp_system_noalloc",
+                     "vs_vertex_normal_maps_inplace"]:
             getattr(L, name).restype = C.c_int
         L.vs_ctx_set_tuning.argtypes = [P, C.c_int, C.c_char_p]
         L.vs_ctx_tuning.argtypes = [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -118,6 +120,7 @@ def lib():
         L.vs_render_maps.argtypes = [P, C.c_int, P, P, P, P, P]
         L.vs_pyramid.argtypes = [P, P]
         L.vs_vertex_normal_maps.argtypes = [P, C.c_int, P, P, P]
+        L.vs_vertex_normal_maps_inplace.argtypes = [P, C.c_int, P, P, P]
         L.vs_track.argtypes = [P, P, P]
         L.vs_integrate.argtypes = [P, P]
         L.vs_raycast.argtypes = [P, C.c_int, P]
@@ -314,6 +317,8 @@ class Context:
 
     # integrate workgroup counts the tuner tries, in the order of vs_ctx_tuning's means
     INTEG_GRIDS = (1024, 2048, 4096, 8192, 16384, 32768)
+    # shading candidates (ray block, w = waves per SIMD the registers aim for; none = 4)
+    SHADE_OPTS = ("32x2", "8x8", "32x2w5", "8x8w5")
 
     def set_tuning(self, mode=VS_TUNE_ONLINE, cache_path=None):
         """Launch-shape autotuning (vs_ctx_set_tuning)."""
@@ -334,7 +339,7 @@ class Context:
             ms = None
             if f[4] != "cached":
                 m, sh, ig = ([float(x) for x in g.split(",")] for g in f[4:7])
-                ms = {"march": dict(zip(["8x8", "16x4", "32x2"], m)), "shade": dict(zip(["32x2", "8x8"], sh)),
+                ms = {"march": dict(zip(["8x8", "16x4", "32x2"], m)), "shade": dict(zip(self.SHADE_OPTS, sh)),
                       "integrate": dict(zip(self.INTEG_GRIDS, ig))}
             out[f[0]] = {"march": f[1], "shade": f[2], "integrate": int(f[3]), "ms": ms}
         return out
