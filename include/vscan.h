@@ -295,7 +295,7 @@ vs_status vs_ctx_stage_ms(vs_ctx *ctx, float ms[8]);
 /* Launch-shape autotuning (no counterpart in the reference, whose launches
  * have fixed shapes). Three launches of a step have interchangeable shapes:
  * the ray block of a wave in the raycast march (8x8, 16x4 or 32x2 rays), the
- * ray block (32x2 or 8x8) and register budget (4 or 5 waves per SIMD) of
+ * ray block (8x8 or 32x2) and register budget (4 or 5 waves per SIMD) of
  * its shading, and the workgroup count of the integrate (1024 to 32768);
  * every shape gives bit-identical results. With
  * VS_TUNE_ONLINE, a batch (vs_fuse_batch, a server's slot table, a session's
@@ -303,17 +303,19 @@ vs_status vs_ctx_stage_ms(vs_ctx *ctx, float ms[8]);
  * and session count rounded up to powers of two, storage types) has no
  * choice yet runs its first 12 steps eagerly with the candidates in turn,
  * timed per stage (those steps ignore vs_ctx_set_timing), then keeps the
- * fastest shape per stage for that class. VS_TUNE_CACHED applies the shapes
+ * fastest shape per stage for that class (a ray block other than the fixed
+ * one only when it is at least 3% faster: the tuning frames are the first
+ * ones, with volumes still filling). VS_TUNE_CACHED applies the shapes
  * settled so far (in this context or read from the cache file) and tunes
  * nothing. VS_TUNE_OFF (the default) tunes nothing and applies nothing: batches
- * not settled earlier use fixed shapes (march 8x8, shade 32x2, integrate 8192
+ * not settled earlier use fixed shapes (march 8x8, shade 8x8, integrate 8192
  * workgroups, 2048 for a session's own steps). cache_path (nullable) names a
  * text file the choices are read from now and written to after each tuning. */
 typedef enum { VS_TUNE_OFF = 0, VS_TUNE_ONLINE = 1, VS_TUNE_CACHED = 2 } vs_tune_mode;
 vs_status vs_ctx_set_tuning(vs_ctx *ctx, int mode, const char *cache_path);
 /* The context's settled shapes, one size class per line:
  * "<class>|<march block>|<shade block>|<integrate workgroups>|<means>", where
- * <means> is "m8x8,m16x4,m32x2|s32x2,s8x8,s32x2w5,s8x8w5|i1024,i2048,i4096,i8192,i16384,i32768" (mean stage
+ * <means> is "m8x8,m16x4,m32x2|s8x8,s32x2,s8x8w5,s32x2w5|i1024,i2048,i4096,i8192,i16384,i32768" (mean stage
  * ms per candidate, -1 for one not timed) or "cached" (from the file). Writes
  * at most len bytes including the terminating zero; *needed (nullable) = the
  * full length + 1. */

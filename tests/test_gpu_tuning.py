@@ -68,7 +68,17 @@ def test_tuned_steps_equal_fixed_shapes(ctx, tmp_path):
             ms = r["ms"]
             assert all(v > 0 for v in ms["march"].values()) and all(v > 0 for v in ms["shade"].values())
             assert all(v > 0 for v in ms["integrate"].values())
-            assert ms["march"][r["march"]] == min(ms["march"].values())
+            # ray blocks: the fastest, unless the first candidate (8x8) is
+            # within 3% of it; integrate grids: the fastest (the reported
+            # means are rounded to 1e-4 ms: a case that close to the 3%
+            # boundary is not decided here)
+            for stage in ("march", "shade"):
+                best = min(ms[stage].values())
+                first = ms[stage]["8x8"]
+                if abs(first - best / (1.0 - 0.03)) < 2e-4:
+                    continue
+                want = "8x8" if first < best / (1.0 - 0.03) else min(ms[stage], key=ms[stage].get)
+                assert r[stage] == want or ms[stage][r[stage]] == ms[stage][want], (stage, r[stage], ms[stage])
             assert ms["integrate"][r["integrate"]] == min(ms["integrate"].values())
     finally:
         for s in tuned + [lone_tuned]:

@@ -348,14 +348,18 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
         float A[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b = 0.0f;
         bool ok = false;
         if (in[g]) ok = icp_summand_d(P, level, xs[g], ys[g], td[g], T21s, A, b);
-        // the wave's 64 rows of this tile; a wave reads only its own rows
-        if (g > 0) __syncthreads();
+        // the wave's 64 rows of this tile. A wave reads only its own rows, and
+        // the LDS performs one wave's accesses in the order they are issued,
+        // so no workgroup barrier is needed between the writes and the reads
+        // (nor before the next tile's writes); the compiler keeps their order
+        // (the rows may alias) and wave_barrier keeps it from moving them.
+        __builtin_amdgcn_wave_barrier();
         float *f = feat[wv][lane];
 #pragma unroll
         for (int k = 0; k < 6; ++k) f[k] = A[k];
         f[6] = b;
         f[7] = ok ? 1.0f : 0.0f;
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             const float a = feat[wv][8 * m + px][fi];

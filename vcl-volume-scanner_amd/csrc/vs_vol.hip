@@ -1367,7 +1367,7 @@ static LevelWork one_level(const WorkItem *work, int n, int level) {
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
     launch_raycast_march(P, lw, tt, wt, nullptr, kRay8x8, s);
-    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay32x2, 4, s);
+    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay8x8, 4, s);
 }
 void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
                           unsigned long long *stats, hipStream_t s) {

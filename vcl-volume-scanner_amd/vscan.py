@@ -318,7 +318,7 @@ class Context:
     # integrate workgroup counts the tuner tries, in the order of vs_ctx_tuning's means
     INTEG_GRIDS = (1024, 2048, 4096, 8192, 16384, 32768)
     # shading candidates (ray block, w = waves per SIMD the registers aim for; none = 4)
-    SHADE_OPTS = ("32x2", "8x8", "32x2w5", "8x8w5")
+    SHADE_OPTS = ("8x8", "32x2", "8x8w5", "32x2w5")
 
     def set_tuning(self, mode=VS_TUNE_ONLINE, cache_path=None):
         """Launch-shape autotuning (vs_ctx_set_tuning)."""
