@@ -1221,7 +1221,9 @@ vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs
 namespace {
 // Batches of at least kSplitMin sessions run as kSplitParts parts in a
 // captured step (64 sessions of 640x480 / 512^3: 10.70 ms per step as one
-// chain, 10.37 as 2 parts, 10.15 as 3, 10.21 as 4).
+// chain, 10.37 as 2 parts, 10.15 as 3, 10.21 as 4; with today's kernels
+// 8.30 as 2, 8.15-8.21 as 3, 8.16-8.17 as 4 and 8.74-8.79 as 5, more
+// branches than the 4 hardware queues a process gets).
 constexpr int kSplitMin = 16;
 #ifndef VS_SPLIT_PARTS
 #define VS_SPLIT_PARTS 3
