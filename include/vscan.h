@@ -381,9 +381,12 @@ vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_sess
  * hbm_budget (0 = no limit); otherwise it returns VS_BUDGET and allocates
  * nothing. vs_server_step fuses one frame for each listed slot in one
  * graph-captured launch sequence; the other open slots keep their volume, pose
- * and maps untouched. Opening or closing a slot waits for the server's steps
- * in flight, then rebuilds the launch sequence once; steps between changes
- * replay it with a per-step active mask. Each session's result is the same as
+ * and maps untouched. Opening or closing a slot does not wait for the steps in
+ * flight: its effects (slot parameters, work lists in the second of two
+ * buffers, freeing a closed session's memory) are enqueued in stream order
+ * after them, and the next step captures the new launch sequence once; steps
+ * between changes replay it with a per-step active mask. Each session's
+ * result is the same as
  * vs_fuse_frame on it alone. Sessions stay usable through vs_server_session
  * (pose, maps, volume download); closing destroys them. */
 typedef struct vs_server vs_server;

@@ -311,6 +311,7 @@ struct vs_session {
     Timer timer;
     vs_server *server = nullptr;  // slot table holding this session, if any
     int slot = -1;
+    bool pooled = false;  // mem from the stream-ordered allocator (slot-table sessions)
 };
 
 // Slot table of the serving form. B spans all max_slots slots: B.dP[i] holds
@@ -330,8 +331,29 @@ struct vs_server {
     char *fixed = nullptr;   // params[max_slots], placeholder states, active flags
     SessState *ghost = nullptr;
     int *active = nullptr;
-    char *lists = nullptr;   // work lists, brick list, brick counter
-    size_t lists_bytes = 0;
+    // Work lists, brick list and brick counter of the current slot layout, in
+    // one of two device buffers: a new layout is written into the buffer the
+    // steps in flight do not read. Each has a pinned host copy the upload is
+    // made from, and an event after that upload.
+    char *lists[2] = {nullptr, nullptr};
+    size_t lists_bytes[2] = {0, 0};
+    char *lstage[2] = {nullptr, nullptr};
+    size_t lstage_bytes[2] = {0, 0};
+    hipEvent_t lstage_ev[2] = {nullptr, nullptr};
+    int gen = 0;  // buffer of the current layout
+    // Slot parameter uploads: a ring of pinned blocks, each with an event
+    // after its upload.
+    SessParams *pstage = nullptr;
+    hipEvent_t *pstage_ev = nullptr;
+    int pstage_n = 0, pstage_next = 0;
+    // Step graphs of earlier layouts, destroyed once the steps that may still
+    // run them have completed (the event after the last of them).
+    struct Retired {
+        hipGraph_t graph;
+        hipGraphExec_t gexec;
+        hipEvent_t done;
+    };
+    std::vector<Retired> retired;
     bool dirty = true;
     unsigned long long bytes = 0;
 };
@@ -365,9 +387,10 @@ struct ListSet {
         }
         return lists_bytes(nt, ng, bricks);
     }
-    // Points the batch at `mem` (bytes() long), uploads the lists and clears
-    // the brick counter; sets the iteration bounds and the integrate grid.
-    vs_status place(Batch &B, char *mem, int integ_max) const {
+    // Points the batch at `mem` (bytes() long): work lists, then the brick
+    // list, then the brick counter; returns the offset of the brick list (the
+    // work lists are the bytes before it).
+    size_t point(Batch &B, char *mem) const {
         size_t off = 0;
         for (int l = 0; l < kMaxLevels; ++l) {
             B.dWork[l] = (WorkItem *)(mem + off);
@@ -376,17 +399,55 @@ struct ListSet {
             B.dGroup[l] = (WorkItem *)(mem + off);
             B.nGroup[l] = (int)group[l].size();
             off = align_up(off + std::max<size_t>(group[l].size(), 1) * sizeof(WorkItem));
+        }
+        const size_t head = off;
+        B.bl.items = (int2 *)(mem + off);
+        off = align_up(off + std::max<size_t>(bricks, 1) * sizeof(int2));
+        B.bl.count = (unsigned *)(mem + off);
+        B.bl.capacity = bricks;
+        return head;
+    }
+    // Bytes of the work lists (the part of the layout a host copy uploads)
+    // plus 16 for the zeroed brick counter.
+    size_t head_bytes() const {
+        size_t off = 0;
+        for (int l = 0; l < kMaxLevels; ++l) {
+            off = align_up(off + std::max<size_t>(tile[l].size(), 1) * sizeof(WorkItem));
+            off = align_up(off + std::max<size_t>(group[l].size(), 1) * sizeof(WorkItem));
+        }
+        return off + 16;
+    }
+    // Points the batch at `mem` (bytes() long), uploads the lists and clears
+    // the brick counter; sets the iteration bounds and the integrate grid.
+    vs_status place(Batch &B, char *mem, int integ_max) const {
+        point(B, mem);
+        for (int l = 0; l < kMaxLevels; ++l) {
             if (!tile[l].empty())
                 HIP_TRY(hipMemcpy(B.dWork[l], tile[l].data(), tile[l].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
             if (!group[l].empty())
                 HIP_TRY(hipMemcpy(B.dGroup[l], group[l].data(), group[l].size() * sizeof(WorkItem),
                                   hipMemcpyHostToDevice));
         }
-        B.bl.items = (int2 *)(mem + off);
-        off = align_up(off + std::max<size_t>(bricks, 1) * sizeof(int2));
-        B.bl.count = (unsigned *)(mem + off);
-        B.bl.capacity = bricks;
         HIP_TRY(hipMemset(B.bl.count, 0, 16));
+        return settle(B, integ_max);
+    }
+    // The same in stream order on h, from `stage` (pinned, head_bytes() long,
+    // not to be rewritten before the uploads have run).
+    vs_status place_async(Batch &B, char *mem, char *stage, hipStream_t h, int integ_max) const {
+        const size_t head = point(B, mem);
+        for (int l = 0; l < kMaxLevels; ++l) {
+            if (!tile[l].empty())
+                std::memcpy(stage + ((char *)B.dWork[l] - mem), tile[l].data(), tile[l].size() * sizeof(WorkItem));
+            if (!group[l].empty())
+                std::memcpy(stage + ((char *)B.dGroup[l] - mem), group[l].data(), group[l].size() * sizeof(WorkItem));
+        }
+        std::memset(stage + head, 0, 16);
+        HIP_TRY(hipMemcpyAsync(mem, stage, head, hipMemcpyHostToDevice, h));
+        HIP_TRY(hipMemcpyAsync(B.bl.count, stage + head, 16, hipMemcpyHostToDevice, h));
+        return settle(B, integ_max);
+    }
+    // Iteration bounds, integrate grid and size class of the batch.
+    vs_status settle(Batch &B, int integ_max) const {
         B.levels = levels;
         B.sd_slices = (max_dim / kBrick + (1 << kSuperShift) - 1) >> kSuperShift;
         for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = max_iters[l];
@@ -444,15 +505,54 @@ void drop_batches_with(vs_ctx *ctx, vs_session *s) {
     }
 }
 
-// Waits for the server's steps in flight (its lists and parameter array are
-// read by them).
+// Waits for the server's steps in flight (used only where the caller needs
+// their results: eviction, destruction).
 vs_status server_quiesce(vs_server *v) {
     if (v->last && v->last != v->stream) HIP_TRY(hipStreamSynchronize(v->last));
     HIP_TRY(hipStreamSynchronize(v->stream));
     return VS_OK;
 }
 
+// The stream whose end follows every step enqueued so far (steps on other
+// streams are chained to their predecessor by an event). Changes to the slot
+// table are enqueued there: they take effect after the steps in flight and
+// before the next one, and the host does not wait.
+hipStream_t server_order(const vs_server *v) { return v->last ? v->last : v->stream; }
+
+// Destroys the step graphs whose last launch has completed (all of them with
+// wait = true, after a quiesce).
+void server_reap(vs_server *v, bool wait) {
+    auto &r = v->retired;
+    for (size_t k = 0; k < r.size();) {
+        if (wait || hipEventQuery(r[k].done) == hipSuccess) {
+            if (r[k].gexec) hipGraphExecDestroy(r[k].gexec);
+            if (r[k].graph) hipGraphDestroy(r[k].graph);
+            hipEventDestroy(r[k].done);
+            r[k] = r.back();
+            r.pop_back();
+        } else {
+            ++k;
+        }
+    }
+}
+
+// Takes the current step graph out of use; it is destroyed after the steps
+// enqueued so far (which may launch it) have completed.
+vs_status server_retire_graph(vs_server *v) {
+    Batch &B = v->B;
+    if (!B.gexec && !B.graph) return VS_OK;
+    vs_server::Retired r{B.graph, B.gexec, nullptr};
+    B.graph = nullptr;
+    B.gexec = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+    v->retired.push_back(r);
+    HIP_TRY(hipEventRecord(r.done, server_order(v)));
+    return VS_OK;
+}
+
 // Parameter block of slot i: the open session's, or the inactive placeholder.
+// Uploaded in stream order after the steps in flight (which may still read
+// the slot's previous block), from a pinned block of the ring.
 vs_status server_write_slot(vs_server *v, int i) {
     SessParams p{};
     if (v->slot[i]) {
@@ -460,46 +560,70 @@ vs_status server_write_slot(vs_server *v, int i) {
     } else {
         p.st = v->ghost + i;
     }
-    HIP_TRY(hipMemcpy(v->B.dP + i, &p, sizeof(p), hipMemcpyHostToDevice));
+    const int k = v->pstage_next;
+    v->pstage_next = (k + 1) % v->pstage_n;
+    HIP_TRY(hipEventSynchronize(v->pstage_ev[k]));  // that block's earlier upload has run
+    v->pstage[k] = p;
+    const hipStream_t h = server_order(v);
+    HIP_TRY(hipMemcpyAsync(v->B.dP + i, v->pstage + k, sizeof(p), hipMemcpyHostToDevice, h));
+    HIP_TRY(hipEventRecord(v->pstage_ev[k], h));
     return VS_OK;
 }
 
-// Removes a session from its slot (the session itself stays alive).
+// Removes a session from its slot (the session itself stays alive); the
+// slot's placeholder block takes effect after the steps in flight.
 vs_status server_forget(vs_session *s) {
     vs_server *v = s->server;
     if (!v) return VS_OK;
-    vs_status r = server_quiesce(v);
     const int i = s->slot;
     v->slot[i] = nullptr;
     v->bytes -= s->L.total;
     s->server = nullptr;
     s->slot = -1;
     v->dirty = true;
-    vs_status w = server_write_slot(v, i);
-    return r != VS_OK ? r : w;
+    return server_write_slot(v, i);
 }
 
 // Work lists (slot, tile) of the open slots, the brick list sized for their
-// bricks, and the iteration bounds of the launch sequence; drops the captured
-// graph so the next step captures the new sequence.
+// bricks, and the iteration bounds of the launch sequence, written into the
+// list buffer the steps in flight do not read, in stream order after them;
+// retires the captured graph so the next step captures the new sequence. The
+// host waits for nothing but, in the rare case of two layout changes within
+// the steps in flight, the upload from the same pinned copy two layouts ago.
 vs_status server_rebuild(vs_server *v) {
     Batch &B = v->B;
-    free_batch_graph(B);
+    vs_status r = server_retire_graph(v);
+    if (r != VS_OK) return r;
     ListSet ls;
     for (int i = 0; i < (int)v->slot.size(); ++i)
         if (v->slot[i]) ls.add(i, v->slot[i]);
     const size_t need = ls.bytes();
-    if (need > v->lists_bytes) {
-        if (v->lists) HIP_TRY(hipFree(v->lists));
-        v->lists = nullptr;
-        v->lists_bytes = 0;
+    const int g = v->gen ^ 1;
+    const hipStream_t h = server_order(v);
+    if (need > v->lists_bytes[g]) {
+        if (v->lists[g]) HIP_TRY(hipFreeAsync(v->lists[g], h));
+        v->lists[g] = nullptr;
+        v->lists_bytes[g] = 0;
         const size_t want = need + need / 2;
-        hipError_t e = hipMalloc((void **)&v->lists, want);
+        hipError_t e = hipMallocAsync((void **)&v->lists[g], want, h);
         if (e != hipSuccess) return fail(VS_ENOMEM, "server lists (%zu bytes): %s", want, hipGetErrorString(e));
-        v->lists_bytes = want;
+        v->lists_bytes[g] = want;
     }
-    vs_status r = ls.place(B, v->lists, 8192);
+    HIP_TRY(hipEventSynchronize(v->lstage_ev[g]));
+    const size_t head = ls.head_bytes();
+    if (head > v->lstage_bytes[g]) {
+        if (v->lstage[g]) HIP_TRY(hipHostFree(v->lstage[g]));
+        v->lstage[g] = nullptr;
+        v->lstage_bytes[g] = 0;
+        const size_t want = head + head / 2;
+        hipError_t e = hipHostMalloc((void **)&v->lstage[g], want, hipHostMallocDefault);
+        if (e != hipSuccess) return fail(VS_ENOMEM, "server list staging (%zu bytes): %s", want, hipGetErrorString(e));
+        v->lstage_bytes[g] = want;
+    }
+    r = ls.place_async(B, v->lists[g], v->lstage[g], h, 8192);
     if (r != VS_OK) return r;
+    HIP_TRY(hipEventRecord(v->lstage_ev[g], h));
+    v->gen = g;
     v->dirty = false;
     return VS_OK;
 }
@@ -520,19 +644,16 @@ vs_status server_admit(vs_server *v, const vs_session_desc *d, int *slot) {
     return VS_OK;
 }
 
-// Puts a new session into free slot i (destroys it on failure).
+// Puts a new session (initialised: its own stream is idle) into free slot i
+// (destroys it on failure). Its parameter block takes effect after the steps
+// in flight.
 vs_status server_attach(vs_server *v, vs_session *s, int i) {
-    vs_status r = server_quiesce(v);
-    if (r != VS_OK) {
-        vs_session_destroy(s);
-        return r;
-    }
     v->slot[i] = s;
     s->server = v;
     s->slot = i;
     v->bytes += s->L.total;
     v->dirty = true;
-    r = server_write_slot(v, i);
+    vs_status r = server_write_slot(v, i);
     if (r != VS_OK) vs_session_destroy(s);
     return r;
 }
@@ -860,7 +981,14 @@ size_t vs_session_bytes(const vs_session_desc *d) {
     return make_layout(*d).total;
 }
 
-vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const double T_gc0[16], vs_session **out) {
+}  // extern "C"
+
+namespace {
+// A session whose memory comes from the stream-ordered allocator when
+// `pooled` (slot-table sessions: closing one frees it in stream order after
+// the steps in flight, where hipFree would wait for the whole device).
+vs_status create_session(vs_ctx *ctx, const vs_session_desc *desc, const double T_gc0[16], bool pooled,
+                         vs_session **out) {
     if (!ctx || !out) return fail(VS_EINVAL, "null argument");
     vs_status cs = check_desc(desc);
     if (cs != VS_OK) return cs;
@@ -870,10 +998,15 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     s->d = *desc;
     s->L = make_layout(*desc);
     const Layout &L = s->L;
-    hipError_t e = hipMalloc((void **)&s->mem, L.total);
-    if (e != hipSuccess) return fail(VS_ENOMEM, "hipMalloc(%zu) failed: %s", L.total, hipGetErrorString(e));
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->last = s->stream;
+    hipError_t e = pooled ? hipMallocAsync((void **)&s->mem, L.total, s->stream) : hipMalloc((void **)&s->mem, L.total);
+    if (e == hipSuccess && pooled) e = hipStreamSynchronize(s->stream);  // usable from any stream from here on
+    if (e != hipSuccess) {
+        hipStreamDestroy(s->stream);
+        return fail(VS_ENOMEM, "device allocation of %zu bytes failed: %s", L.total, hipGetErrorString(e));
+    }
+    s->pooled = pooled;
     char *m = s->mem;
     SessParams &P = s->hp;
     std::memset(&P, 0, sizeof(P));
@@ -993,17 +1126,20 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     }
     B.members = {s.get()};
     HIP_TRY(hipMemcpy(B.dP, &P, sizeof(P), hipMemcpyHostToDevice));
-    // state: pose, counters; volume F = 1, W = 0
-    HIP_TRY(hipMemset(m + L.state, 0, sizeof(SessState)));
+    // state: pose, counters; volume F = 1, W = 0 (vs_session_reset); empty
+    // pyramid and images, model maps at the -100 sentinel; all on the
+    // session's stream (one wait at the end of set-up)
+    HIP_TRY(hipMemsetAsync(m + L.state, 0, sizeof(SessState), s->stream));
+    const uint32_t kMiss = 0xC2C80000u;  // -100.0f
     for (int l = 0; l < kMaxLevels; ++l) {
         const size_t n = (size_t)(d.width >> l) * (d.height >> l);
-        HIP_TRY(hipMemset(m + L.filt[l], 0, n * 4));
-        HIP_TRY(hipMemset(m + L.mimg[l], 0, n));
-        if (P.mtex[l]) HIP_TRY(hipMemset(P.mtex[l], 0, n * 3));
-        std::vector<MapXYZ> inval(n, MapXYZ{-100.f, -100.f, -100.f});
-        HIP_TRY(hipMemcpy(m + L.mvtx[l], inval.data(), n * sizeof(MapXYZ), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(m + L.mnrm[l], inval.data(), n * sizeof(MapXYZ), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(m + L.filt[l], 0, n * 4, s->stream));
+        HIP_TRY(hipMemsetAsync(m + L.mimg[l], 0, n, s->stream));
+        if (P.mtex[l]) HIP_TRY(hipMemsetAsync(P.mtex[l], 0, n * 3, s->stream));
+        launch_fill_u32((uint32_t *)(m + L.mvtx[l]), n * 3, kMiss, s->stream);
+        launch_fill_u32((uint32_t *)(m + L.mnrm[l]), n * 3, kMiss, s->stream);
     }
+    HIP_TRY(hipGetLastError());
     vs_session *raw = s.release();
     ctx->sessions.push_back(raw);
     vs_status r = vs_session_reset(raw, T_gc0, nullptr);
@@ -1015,17 +1151,53 @@ vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const doub
     *out = raw;
     return VS_OK;
 }
+}  // namespace
+
+extern "C" {
+
+vs_status vs_session_create(vs_ctx *ctx, const vs_session_desc *desc, const double T_gc0[16], vs_session **out) {
+    return create_session(ctx, desc, T_gc0, false, out);
+}
 
 vs_status vs_session_destroy(vs_session *s) {
     if (!s) return VS_OK;
     hipSetDevice(s->ctx->device);
-    server_forget(s);
+    if (s->server) {
+        // A slot-table session: its slot takes the placeholder block after
+        // the steps in flight, and its memory is freed in stream order after
+        // them; the host waits for neither.
+        vs_server *v = s->server;
+        server_forget(s);
+        drop_batches_with(s->ctx, s);
+        free_batch_graph(s->self);
+        s->timer.destroy();
+        if (s->stream) hipStreamDestroy(s->stream);
+        if (s->mem) {
+            if (s->pooled) {
+                hipFreeAsync(s->mem, server_order(v));
+            } else {
+                (void)server_quiesce(v);
+                hipFree(s->mem);
+            }
+        }
+        auto &vs = s->ctx->sessions;
+        vs.erase(std::remove(vs.begin(), vs.end(), s), vs.end());
+        delete s;
+        return VS_OK;
+    }
     (void)sync_last(s);
     drop_batches_with(s->ctx, s);
     free_batch_graph(s->self);
     s->timer.destroy();
     if (s->stream) hipStreamDestroy(s->stream);
-    if (s->mem) hipFree(s->mem);
+    if (s->mem) {
+        if (s->pooled) {
+            hipFreeAsync(s->mem, nullptr);
+            hipStreamSynchronize(nullptr);
+        } else {
+            hipFree(s->mem);
+        }
+    }
     auto &v = s->ctx->sessions;
     v.erase(std::remove(v.begin(), v.end(), s), v.end());
     delete s;
@@ -1550,20 +1722,26 @@ static vs_status snapshot_check(const void *buf, size_t bytes, vs_snapshot_heade
     return VS_OK;
 }
 
-vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_session **out) {
+}  // extern "C"
+
+namespace {
+// vs_session_restore; `pooled` as create_session.
+vs_status restore_session(vs_ctx *ctx, const void *buf, size_t bytes, bool pooled, vs_session **out) {
     if (!ctx || !out) return fail(VS_EINVAL, "null argument");
     vs_snapshot_header h;
     vs_status r = snapshot_check(buf, bytes, h);
     if (r != VS_OK) return r;
     vs_session *s = nullptr;
-    r = vs_session_create(ctx, &h.desc, h.T_gc, &s);
+    r = create_session(ctx, &h.desc, h.T_gc, pooled, &s);
     if (r != VS_OK) return r;
     const size_t n = h.bricks_stored;
     const size_t packed = n * (size_t)kBrickVox * (h.value_bytes + h.weight_bytes + h.color_bytes);
     hipError_t e = hipSuccess;
     if (n) {
         char *dev = nullptr;
-        e = hipMalloc((void **)&dev, align_up(n * 4) + packed);
+        e = pooled ? hipMallocAsync((void **)&dev, align_up(n * 4) + packed, s->stream)
+                   : hipMalloc((void **)&dev, align_up(n * 4) + packed);
+        if (e == hipSuccess && pooled) e = hipStreamSynchronize(s->stream);
         if (e == hipSuccess) {
             const char *in = (const char *)buf + sizeof(h);
             e = hipMemcpy(dev, in, n * 4, hipMemcpyHostToDevice);
@@ -1575,7 +1753,11 @@ vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_sess
                 launch_occupancy_rebuild(s->self.dP, s->L.nbricks, s->d.tsdf_type, s->d.weight_type, s->L.sdim[2], s->stream);
                 e = hipStreamSynchronize(s->stream);
             }
-            hipFree(dev);
+            if (pooled) {
+                hipFreeAsync(dev, s->stream);
+            } else {
+                hipFree(dev);
+            }
         }
     }
     if (e == hipSuccess) {
@@ -1596,6 +1778,13 @@ vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_sess
     *out = s;
     return VS_OK;
 }
+}  // namespace
+
+extern "C" {
+
+vs_status vs_session_restore(vs_ctx *ctx, const void *buf, size_t bytes, vs_session **out) {
+    return restore_session(ctx, buf, bytes, false, out);
+}
 
 vs_status vs_server_evict(vs_server *v, int slot, void *buf, size_t capacity, size_t *bytes) {
     if (!v) return fail(VS_EINVAL, "null server");
@@ -1615,7 +1804,7 @@ vs_status vs_server_open_snapshot(vs_server *v, const void *buf, size_t bytes, i
     r = server_admit(v, &h.desc, &i);
     if (r != VS_OK) return r;
     vs_session *s = nullptr;
-    r = vs_session_restore(v->ctx, buf, bytes, &s);
+    r = restore_session(v->ctx, buf, bytes, true, &s);
     if (r != VS_OK) return r;
     r = server_attach(v, s, i);
     if (r != VS_OK) return r;
@@ -1838,6 +2027,12 @@ vs_status vs_server_create(vs_ctx *ctx, const vs_server_desc *desc, vs_server **
     HIP_TRY(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&v->order, hipEventDisableTiming));
     v->last = v->stream;
+    for (int g = 0; g < 2; ++g) HIP_TRY(hipEventCreateWithFlags(&v->lstage_ev[g], hipEventDisableTiming));
+    v->pstage_n = 2 * n + 8;
+    e = hipHostMalloc((void **)&v->pstage, v->pstage_n * sizeof(SessParams), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(VS_ENOMEM, "server parameter staging: %s", hipGetErrorString(e));
+    v->pstage_ev = new hipEvent_t[v->pstage_n];
+    for (int k = 0; k < v->pstage_n; ++k) HIP_TRY(hipEventCreateWithFlags(&v->pstage_ev[k], hipEventDisableTiming));
     for (int i = 0; i < n; ++i) {
         vs_status r = server_write_slot(v.get(), i);
         if (r != VS_OK) {
@@ -1845,6 +2040,7 @@ vs_status vs_server_create(vs_ctx *ctx, const vs_server_desc *desc, vs_server **
             return r;
         }
     }
+    HIP_TRY(hipStreamSynchronize(v->stream));
     ctx->servers.push_back(v.get());
     *out = v.release();
     return VS_OK;
@@ -1856,8 +2052,21 @@ vs_status vs_server_destroy(vs_server *v) {
     (void)server_quiesce(v);
     for (int i = 0; i < (int)v->slot.size(); ++i)
         if (v->slot[i]) vs_session_destroy(v->slot[i]);
+    for (int g = 0; g < 2; ++g)
+        if (v->lists[g]) hipFreeAsync(v->lists[g], server_order(v));
+    (void)server_quiesce(v);
+    server_reap(v, true);
     free_batch_graph(v->B);
-    if (v->lists) hipFree(v->lists);
+    for (int g = 0; g < 2; ++g) {
+        if (v->lstage[g]) hipHostFree(v->lstage[g]);
+        if (v->lstage_ev[g]) hipEventDestroy(v->lstage_ev[g]);
+    }
+    if (v->pstage) hipHostFree(v->pstage);
+    if (v->pstage_ev) {
+        for (int k = 0; k < v->pstage_n; ++k)
+            if (v->pstage_ev[k]) hipEventDestroy(v->pstage_ev[k]);
+        delete[] v->pstage_ev;
+    }
     if (v->fixed) hipFree(v->fixed);
     if (v->order) hipEventDestroy(v->order);
     if (v->stream) hipStreamDestroy(v->stream);
@@ -1873,7 +2082,7 @@ vs_status vs_server_open(vs_server *v, const vs_session_desc *desc, const double
     vs_status r = server_admit(v, desc, &i);
     if (r != VS_OK) return r;
     vs_session *s = nullptr;
-    r = vs_session_create(v->ctx, desc, T_gc0, &s);
+    r = create_session(v->ctx, desc, T_gc0, true, &s);
     if (r != VS_OK) return r;
     r = server_attach(v, s, i);
     if (r != VS_OK) return r;
@@ -1905,9 +2114,9 @@ vs_status vs_server_step(vs_server *v, const int *slots, const uint16_t *const *
         bits[i >> 5] |= 1u << (i & 31);
     }
     HIP_TRY(hipSetDevice(v->ctx->device));
+    server_reap(v, false);
     if (v->dirty) {
-        vs_status r = server_quiesce(v);
-        if (r == VS_OK) r = server_rebuild(v);
+        vs_status r = server_rebuild(v);
         if (r != VS_OK) return r;
     }
     hipStream_t h = stream ? (hipStream_t)stream : v->stream;

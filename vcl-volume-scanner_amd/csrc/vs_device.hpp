@@ -80,6 +80,7 @@ struct alignas(16) SessState {
 struct MapXYZ {
     float x, y, z;
 };
+static_assert(sizeof(MapXYZ) == 12, "model map records are 3 packed floats");
 
 // Immutable per-session parameters plus buffer pointers; one per session in
 // a device array, so a launch can serve a batch of sessions.

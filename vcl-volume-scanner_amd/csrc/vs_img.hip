@@ -269,6 +269,17 @@ __global__ __launch_bounds__(256) void k_gather_depth(const SessParams *__restri
     }
 }
 
+// n 32-bit words at p set to v (session set-up: map sentinels).
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        stg(p, i, v);
+}
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
 void launch_gather_depth(const SessParams *P, const DepthChunk &ch, hipStream_t s) {
     if (ch.n > 0) hipLaunchKernelGGL(k_gather_depth, dim3(64, ch.n), dim3(256), 0, s, P, ch);
 }

@@ -15,6 +15,8 @@ struct DepthChunk {  // passed by value: device frame pointers and the batch slo
     int n;
 };
 void launch_gather_depth(const SessParams *P, const DepthChunk &ch, hipStream_t s);
+// n 32-bit words at p set to v
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void launch_bilateral(const SessParams *P, const WorkItem *work, int n, hipStream_t s);
 void launch_pyr_down(const SessParams *P, const WorkItem *work, int n, int level, hipStream_t s);
 void launch_vnmap(const SessParams *P, const WorkItem *work, int n, int level, float *vtx, float *nrm, hipStream_t s);
