@@ -40,7 +40,8 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
 // Marks the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
 // pixel's ray and appends each newly marked brick once to the batch's list.
 // Neighbouring pixels hit the same few bricks, so a workgroup first
-// deduplicates its up to 6 x 256 samples in an LDS hash set; then each brick
+// deduplicates its up to 6 x 256 samples (first against the left neighbour
+// lane, then in an LDS hash set); then each brick
 // of the set touches the device-wide bitmap once (device-scope atomics are
 // performed past the per-XCD L2 and are the expensive part). The bricks this
 // workgroup marks first are gathered in LDS and appended with one reservation
@@ -64,6 +65,8 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     const float td = (x < L.W && y < L.H) ? raw_depth(P, (size_t)y * L.W + x) : 0.0f;
+    uint32_t bb[6];
+    for (int k = 0; k < 6; ++k) bb[k] = 0xFFFFFFFFu;
     if (td > 0.0f) {
         const M34 Tc = ld_m34(st->T_cg_f);
         const float *Tcg = Tc.m;
@@ -79,7 +82,19 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
             const int by = (tv.y < 0.0f) ? (int)(tv.y - 0.5f) : (int)(tv.y + 0.5f) / kBrick;
             const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)(tv.z + 0.5f) / kBrick;
             if (bx < 0 || bx >= P.bdim[0] || by < 0 || by >= P.bdim[1] || bz < 0 || bz >= P.bdim[2]) break;
-            const uint32_t b = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;
+            bb[n + 4] = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;  // 0xFFFFFFFF: no brick
+        }
+    }
+    // Into the set, sample by sample across the wave: a lane whose brick
+    // equals its left neighbour's for the same sample leaves it to that lane
+    // (runs of ~10 neighbouring pixels share a brick), so far fewer lanes
+    // contend for the same set slot.
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t b = bb[k];
+        const uint32_t left = __shfl_up(b, 1, 64);
+        if (b != 0xFFFFFFFFu && (lane == 0 || b != left)) {
             uint32_t h = (b * 2654435761u) >> 21;  // 11-bit multiplicative hash
             while (true) {
                 const int old = atomicCAS(&set[h], -1, (int)b);
