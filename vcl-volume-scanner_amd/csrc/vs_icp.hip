@@ -163,28 +163,42 @@ __device__ __forceinline__ bool bilerp(const Quad &q, float rx, float ry, f3 &ou
 // reference is evaluated (same FP32 expressions, so the same decisions) and
 // folded into `ok`; the model taps use coordinates clamped into the map, so
 // every load is in range whatever the tests say, and a rejected pixel's
-// Jacobian and residual are zero.
+// Jacobian and residual are zero. In two halves, so that a caller can issue
+// the next pixel's model taps (icp_pre) before it evaluates this pixel's
+// (icp_post); the operations are the same either way.
+struct IcpPre {
+    f3 p;            // the pixel's vertex in the model's frame
+    float rx, ry;    // bilinear weights of the taps
+    bool ok;         // depth valid, projection inside the map
+    Quad qv, qn;     // model vertex and normal taps (loads in flight)
+};
 // td1: the filtered depth at (x, y), loaded by the caller
-__device__ __forceinline__ bool icp_summand_d(const SessParams &P, int l, int x, int y, float td1, const float *T21,
-                                              float A[6], float &b) {
+__device__ __forceinline__ IcpPre icp_pre(const SessParams &P, int l, int x, int y, float td1, const float *T21) {
     const LevelDesc &L = P.lev[l];
     const int W = L.W, H = L.H;
-    const f3 p = transform(T21, back_project((float)x, (float)y, L.K, td1));
+    IcpPre r;
+    r.p = transform(T21, back_project((float)x, (float)y, L.K, td1));
     float u, v;
-    project(p, L.K, u, v);
+    project(r.p, L.K, u, v);
     const bool inside = u >= 0.0f && u <= float(W - 1) && v >= 0.0f && v <= float(H - 1);
-    bool ok = !(td1 < 1.0e-6f) && inside;
+    r.ok = !(td1 < 1.0e-6f) && inside;
     const float uc = inside ? u : 0.0f, vc = inside ? v : 0.0f;
     const int x0 = (int)uc, y0 = (int)vc;
-    const float rx = uc - (float)x0, ry = vc - (float)y0;
-    const size_t i0 = (size_t)y0 * W + x0, di = rx > 0.0f ? 1 : 0, dj = ry > 0.0f ? (size_t)W : 0;
-    const Quad qv = load_quad(P.mvtx[l], i0, di, dj);
-    const Quad qn = load_quad(P.mnrm[l], i0, di, dj);
+    r.rx = uc - (float)x0;
+    r.ry = vc - (float)y0;
+    const size_t i0 = (size_t)y0 * W + x0, di = r.rx > 0.0f ? 1 : 0, dj = r.ry > 0.0f ? (size_t)W : 0;
+    r.qv = load_quad(P.mvtx[l], i0, di, dj);
+    r.qn = load_quad(P.mnrm[l], i0, di, dj);
+    return r;
+}
+__device__ __forceinline__ bool icp_post(const SessParams &P, const IcpPre &r, float A[6], float &b) {
+    const f3 p = r.p;
+    bool ok = r.ok;
     f3 v0, n;
-    ok &= bilerp(qv, rx, ry, v0);
+    ok &= bilerp(r.qv, r.rx, r.ry, v0);
     const float dx = v0.x - p.x, dy = v0.y - p.y, dz = v0.z - p.z;
     ok &= !(dx * dx + dy * dy + dz * dz > P.icp_dist2);
-    ok &= bilerp(qn, rx, ry, n);
+    ok &= bilerp(r.qn, r.rx, r.ry, n);
     n = normalised(n);
     A[0] = ok ? p.z * n.y + (-p.y) * n.z : 0.0f;
     A[1] = ok ? (-p.z) * n.x + p.x * n.z : 0.0f;
@@ -194,6 +208,10 @@ __device__ __forceinline__ bool icp_summand_d(const SessParams &P, int l, int x,
     A[5] = ok ? -n.z : 0.0f;
     b = ok ? n.x * dx + n.y * dy + n.z * dz : 0.0f;
     return ok;
+}
+__device__ __forceinline__ bool icp_summand_d(const SessParams &P, int l, int x, int y, float td1, const float *T21,
+                                              float A[6], float &b) {
+    return icp_post(P, icp_pre(P, l, x, y, td1, T21), A, b);
 }
 __device__ __forceinline__ bool icp_summand(const SessParams &P, int l, int x, int y, const float *T21, float A[6],
                                             float &b) {
@@ -343,11 +361,16 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int px = ((lane & 15) >> 3) * 4 + (lane >> 4), fi = lane & 7;
     vs_f4x g4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    // one pixel ahead: the next tile's model taps are in flight while this
+    // tile's summand is evaluated (a pixel outside the frame has depth 0 and
+    // adds zeros)
+    IcpPre nxt = icp_pre(P, level, xs[0], ys[0], in[0] ? td[0] : 0.0f, T21s);
 #pragma unroll
     for (int g = 0; g < kIcpGroup; ++g) {
         float A[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b = 0.0f;
-        bool ok = false;
-        if (in[g]) ok = icp_summand_d(P, level, xs[g], ys[g], td[g], T21s, A, b);
+        const IcpPre cur = nxt;
+        if (g + 1 < kIcpGroup) nxt = icp_pre(P, level, xs[g + 1], ys[g + 1], in[g + 1] ? td[g + 1] : 0.0f, T21s);
+        const bool ok = icp_post(P, cur, A, b);
         // the wave's 64 rows of this tile. A wave reads only its own rows, and
         // the LDS performs one wave's accesses in the order they are issued,
         // so no workgroup barrier is needed between the writes and the reads
