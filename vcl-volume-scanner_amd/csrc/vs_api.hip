@@ -65,7 +65,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
+    size_t tsdf, weight, bvis, vmask, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t wtab;                          // voxel-centre world coordinates, x then y then z
     size_t total;
@@ -121,6 +121,7 @@ Layout make_layout(const vs_session_desc &d) {
     L.tsdf = take(L.nvox * sv);
     L.weight = take(L.nvox * sw);
     L.bvis = take((size_t)L.nwords * 4);
+    L.vmask = take(((size_t)L.nbricks + 127) / 128 * 128);
     L.bocc = take((size_t)L.nwords * 4);
     L.bband = take((size_t)L.nwords * 4);
     L.sband = take((size_t)L.nswords * 4);
@@ -133,7 +134,7 @@ Layout make_layout(const vs_session_desc &d) {
         size_t nt[kMaxLevels], ng[kMaxLevels];
         for (int l = 0; l < kMaxLevels; ++l) {
             nt[l] = (size_t)L.tiles[l];
-            ng[l] = (size_t)icp_groups(L.tiles[l]);
+            ng[l] = (size_t)icp_groups(L.tiles[l], l);
         }
         L.lists = take(lists_bytes(nt, ng, L.nbricks));
     }
@@ -265,6 +266,7 @@ struct Batch {
     int nGroup[kMaxLevels] = {0, 0, 0};
     BrickList bl{};
     uint32_t bricks = 0;    // capacity of the brick list
+    uint32_t max_words = 0; // longest visible-brick bitmap of a member (32-bit words)
     int integ_grid = 1;     // default integrate grid
     Shape shape;
     std::string size_class;
@@ -363,11 +365,12 @@ namespace {
 // Host copy of a batch's work lists (see lists_bytes for the device layout).
 struct ListSet {
     std::vector<WorkItem> tile[kMaxLevels], group[kMaxLevels];
-    uint32_t bricks = 0;
+    uint32_t bricks = 0, max_words = 0;
     int levels = 0, max_iters[kMaxLevels] = {0, 0, 0};
     int count = 0, max_w = 0, max_h = 0, max_dim = 0;
     void add(int slot, const vs_session *s) {
         ++count;
+        max_words = std::max(max_words, s->L.nwords);
         max_w = std::max(max_w, s->d.width);
         max_h = std::max(max_h, s->d.height);
         for (int k = 0; k < 3; ++k) max_dim = std::max(max_dim, s->d.vol_dim[k]);
@@ -375,7 +378,7 @@ struct ListSet {
         for (int l = 0; l < kMaxLevels; ++l) {
             max_iters[l] = std::max(max_iters[l], s->hp.iters[l]);
             for (int t = 0; t < s->L.tiles[l]; ++t) tile[l].push_back({slot, t});
-            for (int g = 0; g < icp_groups(s->L.tiles[l]); ++g) group[l].push_back({slot, g});
+            for (int g = 0; g < icp_groups(s->L.tiles[l], l); ++g) group[l].push_back({slot, g});
         }
         bricks += s->L.nbricks;
     }
@@ -452,6 +455,7 @@ struct ListSet {
         B.sd_slices = (max_dim / kBrick + (1 << kSuperShift) - 1) >> kSuperShift;
         for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = max_iters[l];
         B.bricks = bricks;
+        B.max_words = max_words;
         B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(bricks, (uint32_t)integ_max));
         // size class: largest frame, largest volume edge and session count
         // rounded up to powers of two, storage types
@@ -683,7 +687,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     launch_track_finalize(B.dP, B.n, s);
     mark(3);
     launch_vis_clear(B.dP, B.n, B.bl, s);
-    launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, s);
+    launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.n, B.max_words, B.bl, s);
     mark(4);
     launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), B.sd_slices, s);
     mark(5);
@@ -1077,6 +1081,7 @@ vs_status create_session(vs_ctx *ctx, const vs_session_desc *desc, const double 
     P.tsdf = m + L.tsdf;
     P.weight = m + L.weight;
     P.bvis = (uint32_t *)(m + L.bvis);
+    P.vmask = (uint8_t *)(m + L.vmask);
     P.bocc = (uint32_t *)(m + L.bocc);
     P.bband = (uint32_t *)(m + L.bband);
     P.sband = (uint32_t *)(m + L.sband);
@@ -1347,7 +1352,7 @@ vs_status vs_integrate(vs_session *s, vs_stream stream) {
     const Batch &B = s->self;
     launch_fuse_ok(B.dP, 1, h);
     launch_vis_clear(B.dP, 1, B.bl, h);
-    launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], B.bl, h);
+    launch_vis_mark(B.dP, B.dWork[0], B.nWork[0], 1, B.max_words, B.bl, h);
     launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), B.sd_slices, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;

@@ -21,11 +21,22 @@ constexpr int kBrickVox = 512;
 constexpr int kTileW = 32;          // image tile of one 256-thread workgroup: 32 x 8 pixels
 constexpr int kTileH = 8;
 constexpr int kTileThreads = kTileW * kTileH;
-constexpr int kIcpVals = 29;
-// ICP tiles summed by one workgroup (a fixed grouping of each session's
-// tiles, so the sums do not depend on the batch)
-constexpr int kIcpGroup = 4;
-__host__ __device__ constexpr int icp_groups(int tiles) { return (tiles + kIcpGroup - 1) / kIcpGroup; }        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
+constexpr int kIcpVals = 29;        // 21 JtJ (upper triangle) + 6 Jtr + count + squared residual
+// ICP tiles summed by one workgroup, per pyramid level: a fixed grouping of
+// each session's tiles, so the sums do not depend on the batch.
+#ifndef VS_ICP_GROUP0
+#define VS_ICP_GROUP0 4
+#endif
+#ifndef VS_ICP_GROUP1
+#define VS_ICP_GROUP1 4
+#endif
+#ifndef VS_ICP_GROUP2
+#define VS_ICP_GROUP2 4
+#endif
+__host__ __device__ constexpr int icp_group(int level) {
+    return level == 0 ? VS_ICP_GROUP0 : (level == 1 ? VS_ICP_GROUP1 : VS_ICP_GROUP2);
+}
+__host__ __device__ constexpr int icp_groups(int tiles, int level) { return (tiles + icp_group(level) - 1) / icp_group(level); }
 // Superbricks: cubes of 2^3 bricks (16^3 voxels). The raycast's distance
 // field (below) is kept per superbrick and copied to each raycast
 // workgroup's LDS, whose 16 KB hold the 32^3 superbricks of a 512^3 grid;
@@ -114,6 +125,8 @@ struct alignas(16) SessParams {
     void *tsdf;              // brick-major values
     void *weight;            // brick-major weights
     uint32_t *bvis;          // visible-brick bits of the current frame
+    uint8_t *vmask;          // visible-brick flags of the current frame, one byte per brick
+                             // (zero-padded to a multiple of 128 bricks)
     uint32_t *bocc;          // bricks holding any weight > 0 (monotone)
     // Band bits (set-only, so a superset): a brick whose bit is clear holds
     // only voxels of weight 0 or value exactly 1.0, which the raycast's march
@@ -197,6 +210,7 @@ __device__ __forceinline__ bool roundf_ref(float v, int &out) {
 #define VS_GLOBAL __attribute__((address_space(1)))
 #define VS_LDS __attribute__((address_space(3)))
 typedef float vs_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vs_u4 __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ T ldg(const T *p, size_t i) {
     return ((const VS_GLOBAL T *)p)[i];

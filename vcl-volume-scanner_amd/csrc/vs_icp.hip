@@ -315,7 +315,7 @@ __device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *s
 }
 
 // One Gauss-Newton iteration at `level`, first half: the 29 sums of every
-// group of kIcpGroup consecutive 32x8 tiles of every session of the work
+// group of G = icp_group(level) consecutive 32x8 tiles of every session of the work
 // list (items are (session, group)), one FP64 partial per group, summed on
 // the matrix cores (above) or, with VS_ICP_MFMA=0, by per-thread FP32
 // accumulation over the group's tiles and one block reduction (block_terms).
@@ -323,6 +323,7 @@ __device__ __forceinline__ bool icp_skip(const SessParams &P, const SessState *s
 // multi-XCD device an agent-scope release per workgroup would write back L2
 // each time, so the reduction and the solve are the next launch
 // (k_icp_solve).
+template <int G>
 __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__restrict__ SP,
                                                            const WorkItem *__restrict__ work, int level, int iter) {
     __shared__ float T21s[16];
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     if (level >= P.levels || icp_skip(P, st, level, iter)) return;
     const LevelDesc &L = P.lev[level];
     const int ntiles = L.tiles_x * L.tiles_y;
-    const int t0 = wi.tile * kIcpGroup;
+    const int t0 = wi.tile * G;
     if (t0 >= ntiles) return;
     if (threadIdx.x < 16) T21s[threadIdx.x] = ldg(st->T_21_f, threadIdx.x);
     __syncthreads();
@@ -344,11 +345,11 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
 #endif
     // the group's depths first (independent loads; a pixel outside the
     // frame re-reads pixel 0 and is skipped), then the pixels in order
-    int xs[kIcpGroup], ys[kIcpGroup];
-    bool in[kIcpGroup];
-    float td[kIcpGroup];
+    int xs[G], ys[G];
+    bool in[G];
+    float td[G];
 #pragma unroll
-    for (int g = 0; g < kIcpGroup; ++g) {
+    for (int g = 0; g < G; ++g) {
         const int tile = t0 + g;
         xs[g] = (tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
         ys[g] = (tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
@@ -366,10 +367,10 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     // adds zeros)
     IcpPre nxt = icp_pre(P, level, xs[0], ys[0], in[0] ? td[0] : 0.0f, T21s);
 #pragma unroll
-    for (int g = 0; g < kIcpGroup; ++g) {
+    for (int g = 0; g < G; ++g) {
         float A[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b = 0.0f;
         const IcpPre cur = nxt;
-        if (g + 1 < kIcpGroup) nxt = icp_pre(P, level, xs[g + 1], ys[g + 1], in[g + 1] ? td[g + 1] : 0.0f, T21s);
+        if (g + 1 < G) nxt = icp_pre(P, level, xs[g + 1], ys[g + 1], in[g + 1] ? td[g + 1] : 0.0f, T21s);
         const bool ok = icp_post(P, cur, A, b);
         // the wave's 64 rows of this tile. A wave reads only its own rows, and
         // the LDS performs one wave's accesses in the order they are issued,
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
     }
 #else
 #pragma unroll
-    for (int g = 0; g < kIcpGroup; ++g) {
+    for (int g = 0; g < G; ++g) {
         if (in[g]) {
             float A[6], b;
             const bool ok = icp_summand_d(P, level, xs[g], ys[g], td[g], T21s, A, b);
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
     SessState *st = P.st;
     if (level >= P.levels || icp_skip(P, st, level, iter)) return;
     const LevelDesc &L = P.lev[level];
-    const int ngroups = icp_groups(L.tiles_x * L.tiles_y);
+    const int ngroups = icp_groups(L.tiles_x * L.tiles_y, level);
     double acc[kIcpVals];
 #pragma unroll
     for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0;
@@ -658,7 +659,15 @@ void launch_set_active(int *active, int nslots, const uint32_t bits[kMaxSlots / 
 void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess, int level, int iter,
                      hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_icp_iter, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
+    static_assert(icp_group(0) == 4 || icp_group(0) == 2 || icp_group(0) == 1 || icp_group(0) == 8, "group size");
+    static_assert(icp_group(1) == 4 || icp_group(1) == 2 || icp_group(1) == 1 || icp_group(1) == 8, "group size");
+    static_assert(icp_group(2) == 4 || icp_group(2) == 2 || icp_group(2) == 1 || icp_group(2) == 8, "group size");
+    switch (icp_group(level)) {
+    case 1: hipLaunchKernelGGL(k_icp_iter<1>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
+    case 2: hipLaunchKernelGGL(k_icp_iter<2>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
+    case 4: hipLaunchKernelGGL(k_icp_iter<4>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
+    default: hipLaunchKernelGGL(k_icp_iter<8>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
+    }
     hipLaunchKernelGGL(k_icp_solve, dim3(nsess), dim3(kTileThreads), 0, s, P, nsess, level, iter);
 }
 void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s) {

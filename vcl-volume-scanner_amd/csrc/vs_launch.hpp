@@ -45,7 +45,10 @@ struct BrickList {
     unsigned capacity;
 };
 void launch_vis_clear(const SessParams *P, int nsess, BrickList bl, hipStream_t s);
-void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList bl, hipStream_t s);
+// marks the visible bricks' bits, then compacts every session's bits into
+// the brick list (max_words: the longest bitmap of the nsess sessions)
+void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, int nsess, uint32_t max_words, BrickList bl,
+                     hipStream_t s);
 // sd_slices: the most superbricks along z of any session (distance field)
 void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_type, int weight_type, int grid,
                       int sd_slices, hipStream_t s);

@@ -9,7 +9,7 @@
 // stencils of the raycast mostly stay inside one or two bricks.
 //
 // Per-session bitmaps: `bvis` (bricks touched by the current frame, rebuilt
-// each frame), `bocc` (bricks holding any voxel of weight > 0; weights never
+// each frame from the per-brick flags `vmask`), `bocc` (bricks holding any voxel of weight > 0; weights never
 // decrease, so the bit is set once and kept) and the band bits `bband` /
 // `sband` (bricks / superbricks that may hold a voxel of weight > 0 and a value
 // other than 1.0; set when an update writes such a value, and kept).
@@ -21,6 +21,8 @@ namespace vs {
 // ---------------------------------------------------------------------------
 // A13: visible bricks (g_fvsc_Find_Visible_Sub_Cubes, volume_integrator.cu:84-152)
 // ---------------------------------------------------------------------------
+// Clears the list counter and, for every session that fuses this frame, its
+// visible-brick flags.
 __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickList bl) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *bl.count = 0u;
     const int s = blockIdx.y;
@@ -33,35 +35,27 @@ __global__ void k_vis_clear(const SessParams *__restrict__ SP, int nsess, BrickL
     }
     if (!P.st->fuse_ok) return;
     const uint32_t nb = (uint32_t)P.bdim[0] * P.bdim[1] * P.bdim[2];
-    const uint32_t nw = (nb + 31) / 32;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) stg(P.bvis, w, 0u);
+    const uint32_t n16 = (nb + 127) / 128 * 8;  // 16-byte words of the padded flags
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n16; w += gridDim.x * blockDim.x)
+        stg((vs_u4 *)P.vmask, w, vs_u4{0u, 0u, 0u, 0u});
 }
 
-// Marks the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
-// pixel's ray and appends each newly marked brick once to the batch's list.
-// Neighbouring pixels hit the same few bricks, so a workgroup first
-// deduplicates its up to 6 x 256 samples (first against the left neighbour
-// lane, then in an LDS hash set); then each brick
-// of the set touches the device-wide bitmap once (device-scope atomics are
-// performed past the per-XCD L2 and are the expensive part). The bricks this
-// workgroup marks first are gathered in LDS and appended with one reservation
-// on the list counter (one same-address atomic per workgroup, not per brick).
-constexpr int kVisSet = 2048;   // > 6 samples x 256 pixels, so probing always ends
-constexpr int kVisNew = 6 * kTileThreads;
-
+// Flags the bricks hit by the samples p + n*mu*dir, n = -4..1, of every valid
+// pixel's ray: one byte per brick, set to 1 by plain stores (the reference's
+// bool mask), so the bricks that many pixels hit need neither atomics nor a
+// deduplication pass; a lane whose brick equals its left neighbour's for the
+// same sample leaves the store to that lane (runs of ~10 neighbouring pixels
+// share a brick). The brick list and the visible-brick bits are built from
+// the flags by k_vis_list. (A device-scope atomic per new brick on a bitmap,
+// and a returning atomic on one list counter per image tile, had cost this
+// launch most of its time.)
 __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__restrict__ SP,
-                                                           const WorkItem *__restrict__ work, BrickList bl) {
-    __shared__ int set[kVisSet];
-    __shared__ int fresh[kVisNew];
-    __shared__ unsigned nfresh, base;
+                                                           const WorkItem *__restrict__ work) {
     const WorkItem wi = work[blockIdx.x];
     const SessParams &P = SP[wi.sess];
     SessState *st = P.st;
     const LevelDesc &L = P.lev[0];
     if (!st->fuse_ok || tile_out(L, wi.tile)) return;
-    for (int i = threadIdx.x; i < kVisSet; i += kTileThreads) set[i] = -1;
-    if (threadIdx.x == 0) nfresh = 0u;
-    __syncthreads();
     const int x = (wi.tile % L.tiles_x) * kTileW + threadIdx.x % kTileW;
     const int y = (wi.tile / L.tiles_x) * kTileH + threadIdx.x / kTileW;
     const float td = (x < L.W && y < L.H) ? raw_depth(P, (size_t)y * L.W + x) : 0.0f;
@@ -85,62 +79,89 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
             bb[n + 4] = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;  // 0xFFFFFFFF: no brick
         }
     }
-    // Into the set, sample by sample across the wave: a lane whose brick
-    // equals its left neighbour's for the same sample leaves it to that lane
-    // (runs of ~10 neighbouring pixels share a brick), so far fewer lanes
-    // contend for the same set slot.
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const uint32_t b = bb[k];
         const uint32_t left = __shfl_up(b, 1, 64);
-        if (b != 0xFFFFFFFFu && (lane == 0 || b != left)) {
-            uint32_t h = (b * 2654435761u) >> 21;  // 11-bit multiplicative hash
-            while (true) {
-                const int old = atomicCAS(&set[h], -1, (int)b);
-                if (old == -1 || old == (int)b) break;
-                h = (h + 1) & (kVisSet - 1);
+        if (b != 0xFFFFFFFFu && (lane == 0 || b != left)) stg(P.vmask, b, (uint8_t)1);
+    }
+}
+
+// Turns the sessions' visible-brick flags into the visible-brick bits and
+// the batch's brick list. Workgroup (c, s) takes bricks [c, c + 1) *
+// kVisListBricks of session s: each thread packs the flags of 128
+// consecutive bricks (eight 16-byte loads) into 4 bit words and stores them,
+// then the workgroup ranks the set bits (popcounts and a prefix sum),
+// reserves their range with one atomic on the list counter (8 workgroups for
+// a 512^3 session) and writes (session, brick) in brick order, each thread
+// its own bricks.
+constexpr int kVisListWordsPerThread = 4;
+constexpr uint32_t kVisListBricks = 32u * kVisListWordsPerThread * kTileThreads;
+
+__global__ __launch_bounds__(kTileThreads) void k_vis_list(const SessParams *__restrict__ SP, BrickList bl) {
+    __shared__ unsigned wsum[kTileThreads / 64];
+    __shared__ unsigned base;
+    const int sess = blockIdx.y;
+    const SessParams &P = SP[sess];
+    SessState *st = P.st;
+    if (!ldg(&st->fuse_ok, 0)) return;
+    const uint32_t nb = (uint32_t)P.bdim[0] * P.bdim[1] * P.bdim[2], nw = (nb + 31) / 32;
+    const uint32_t b0 = blockIdx.x * kVisListBricks + threadIdx.x * (32u * kVisListWordsPerThread);
+    if (blockIdx.x * kVisListBricks >= nb) return;
+    uint32_t w[kVisListWordsPerThread] = {0u, 0u, 0u, 0u};
+    if (b0 < nb) {  // flags padded to 128 bricks: the 128 read here are in the buffer
+        const vs_u4 *f = (const vs_u4 *)(P.vmask + b0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const vs_u4 v = ldg(f, q);
+            const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t x = e[r];  // flags of bricks b0 + 16q + 4r .. +3, one byte each (0 or 1)
+                const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+                w[q >> 1] |= nib << (16 * (q & 1) + 4 * r);
             }
         }
     }
-    __syncthreads();
-    // Set the device bits of the workgroup's bricks: each thread takes
-    // kVisSet / kTileThreads slots and issues their hint reads, then their
-    // returning atomics, together (device-scope atomics are slow; waiting
-    // for each before issuing the next made them a serial chain). Bits are
-    // only set during this launch (cleared by the launch before): a plain
-    // cached read that shows the bit proves it set; a stale 0 costs one atomic.
-    {
-        constexpr int kPer = kVisSet / kTileThreads;
-        int bs[kPer];
-        uint32_t hint[kPer], got[kPer];
+    const uint32_t w0 = b0 / 32u;
+    unsigned c = 0;
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            bs[j] = set[threadIdx.x + j * kTileThreads];
-            hint[j] = bs[j] >= 0 ? ldg(P.bvis, (uint32_t)bs[j] >> 5) : 0xFFFFFFFFu;
-        }
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const uint32_t bit = bs[j] >= 0 ? 1u << ((uint32_t)bs[j] & 31u) : 1u;
-            got[j] = (hint[j] & bit) ? bit : atomicOr(P.bvis + ((uint32_t)bs[j] >> 5), bit);
-        }
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const uint32_t bit = bs[j] >= 0 ? 1u << ((uint32_t)bs[j] & 31u) : 1u;
-            if (bs[j] >= 0 && !(got[j] & bit)) fresh[atomicAdd(&nfresh, 1u)] = bs[j];
-        }
+    for (int k = 0; k < kVisListWordsPerThread; ++k) {
+        if (w0 + k < nw) stg(P.bvis, w0 + k, w[k]);
+        c += (unsigned)__popc(w[k]);
     }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned inc = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned v = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += v;
+    }
+    if (lane == 63) wsum[wv] = inc;
     __syncthreads();
-    const unsigned nf = nfresh;
-    if (nf == 0u) return;
+    unsigned before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < kTileThreads / 64; ++k) {
+        before += (k < wv) ? wsum[k] : 0u;
+        total += wsum[k];
+    }
+    if (total == 0) return;
     if (threadIdx.x == 0) {
-        base = atomicAdd(bl.count, nf);
-        atomicAdd(&st->vis_count, nf);
+        base = atomicAdd(bl.count, total);
+        atomicAdd(&st->vis_count, total);
     }
     __syncthreads();
-    for (unsigned i = threadIdx.x; i < nf; i += kTileThreads) {
-        const unsigned idx = base + i;
-        if (idx < bl.capacity) bl.items[idx] = make_int2(wi.sess, fresh[i]);
+    unsigned pos = base + before + inc - c;
+#pragma unroll
+    for (int k = 0; k < kVisListWordsPerThread; ++k) {
+        uint32_t bits = w[k];
+        while (bits) {
+            const int j = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            if (pos < bl.capacity) bl.items[pos] = make_int2(sess, (int)((w0 + k) * 32u + (uint32_t)j));
+            ++pos;
+        }
     }
 }
 
@@ -1295,8 +1316,12 @@ __global__ __launch_bounds__(kTileThreads) void k_occupancy(const SessParams *__
 void launch_vis_clear(const SessParams *P, int nsess, BrickList bl, hipStream_t s) {
     hipLaunchKernelGGL(k_vis_clear, dim3(8, nsess), dim3(256), 0, s, P, nsess, bl);
 }
-void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, BrickList bl, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_vis_mark, dim3(n), dim3(kTileThreads), 0, s, P, work, bl);
+void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, int nsess, uint32_t max_words, BrickList bl,
+                     hipStream_t s) {
+    if (n <= 0 || nsess <= 0 || max_words == 0) return;
+    hipLaunchKernelGGL(k_vis_mark, dim3(n), dim3(kTileThreads), 0, s, P, work);
+    const uint32_t per = kVisListBricks / 32u;  // bitmap words per k_vis_list workgroup
+    hipLaunchKernelGGL(k_vis_list, dim3((max_words + per - 1) / per, nsess), dim3(kTileThreads), 0, s, P, bl);
 }
 
 #define VS_DISPATCH(tt, wt, FN, ...)                                                \
