@@ -64,7 +64,7 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
-    size_t state, raw, filt[kMaxLevels], mvtx[kMaxLevels], mnrm[kMaxLevels], mimg[kMaxLevels];
+    size_t state, raw, filt[kMaxLevels], mvn[kMaxLevels], mimg[kMaxLevels];
     size_t tsdf, weight, bvis, vmask, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t wtab;                          // voxel-centre world coordinates, x then y then z
@@ -114,8 +114,7 @@ Layout make_layout(const vs_session_desc &d) {
         const size_t n = (size_t)Wl * Hl;
         L.tiles[l] = l < d.levels ? tiles_of(Wl, Hl) : 0;
         L.filt[l] = take(n * 4);
-        L.mvtx[l] = take(n * sizeof(MapXYZ));
-        L.mnrm[l] = take(n * sizeof(MapXYZ));
+        L.mvn[l] = take((n + 1) * 2 * sizeof(MapXYZ));  // vertex + normal per pixel, one padding pixel
         L.mimg[l] = take(n);
     }
     L.tsdf = take(L.nvox * sv);
@@ -1074,8 +1073,7 @@ vs_status create_session(vs_ctx *ctx, const vs_session_desc *desc, const double 
     P.raw = m + L.raw;
     for (int l = 0; l < kMaxLevels; ++l) {
         P.filt[l] = (float *)(m + L.filt[l]);
-        P.mvtx[l] = (MapXYZ *)(m + L.mvtx[l]);
-        P.mnrm[l] = (MapXYZ *)(m + L.mnrm[l]);
+        P.mvn[l] = (MapXYZ *)(m + L.mvn[l]);
         P.mimg[l] = (uint8_t *)(m + L.mimg[l]);
     }
     P.tsdf = m + L.tsdf;
@@ -1141,8 +1139,7 @@ vs_status create_session(vs_ctx *ctx, const vs_session_desc *desc, const double 
         HIP_TRY(hipMemsetAsync(m + L.filt[l], 0, n * 4, s->stream));
         HIP_TRY(hipMemsetAsync(m + L.mimg[l], 0, n, s->stream));
         if (P.mtex[l]) HIP_TRY(hipMemsetAsync(P.mtex[l], 0, n * 3, s->stream));
-        launch_fill_u32((uint32_t *)(m + L.mvtx[l]), n * 3, kMiss, s->stream);
-        launch_fill_u32((uint32_t *)(m + L.mnrm[l]), n * 3, kMiss, s->stream);
+        launch_fill_u32((uint32_t *)(m + L.mvn[l]), (n + 1) * 6, kMiss, s->stream);
     }
     HIP_TRY(hipGetLastError());
     vs_session *raw = s.release();
@@ -1547,13 +1544,16 @@ vs_status vs_get_maps(vs_session *s, int level, float *depth, float *vtx, float 
     vs_status r = sync_last(s);
     if (r != VS_OK) return r;
     const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
-    // the device maps are the ABI's layout (x, y, z per pixel)
-    std::vector<MapXYZ> v(n);
-    HIP_TRY(hipMemcpy(v.data(), s->hp.mvtx[level], n * sizeof(MapXYZ), hipMemcpyDeviceToHost));
-    if (vtx) std::memcpy(vtx, v.data(), n * sizeof(MapXYZ));
-    if (nrm) HIP_TRY(hipMemcpy(nrm, s->hp.mnrm[level], n * sizeof(MapXYZ), hipMemcpyDeviceToHost));
-    if (depth)
-        for (size_t i = 0; i < n; ++i) depth[i] = (v[i].x == -100.0f) ? 0.0f : v[i].z;
+    // the device maps interleave vertex and normal records; the ABI's maps
+    // are x, y, z per pixel, one map each
+    std::vector<MapXYZ> v(2 * n);
+    HIP_TRY(hipMemcpy(v.data(), s->hp.mvn[level], 2 * n * sizeof(MapXYZ), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        const MapXYZ a = v[2 * i], b = v[2 * i + 1];
+        if (vtx) std::memcpy(vtx + 3 * i, &a, sizeof(MapXYZ));
+        if (nrm) std::memcpy(nrm + 3 * i, &b, sizeof(MapXYZ));
+        if (depth) depth[i] = (a.x == -100.0f) ? 0.0f : a.z;
+    }
     if (img) HIP_TRY(hipMemcpy(img, s->hp.mimg[level], n, hipMemcpyDeviceToHost));
     return VS_OK;
 }
@@ -1564,8 +1564,12 @@ vs_status vs_set_maps(vs_session *s, int level, const float *vtx, const float *n
     vs_status r = sync_last(s);
     if (r != VS_OK) return r;
     const size_t n = (size_t)(s->d.width >> level) * (s->d.height >> level);
-    HIP_TRY(hipMemcpy(s->hp.mvtx[level], vtx, n * sizeof(MapXYZ), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->hp.mnrm[level], nrm, n * sizeof(MapXYZ), hipMemcpyHostToDevice));
+    std::vector<MapXYZ> v(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        std::memcpy(&v[2 * i], vtx + 3 * i, sizeof(MapXYZ));
+        std::memcpy(&v[2 * i + 1], nrm + 3 * i, sizeof(MapXYZ));
+    }
+    HIP_TRY(hipMemcpy(s->hp.mvn[level], v.data(), 2 * n * sizeof(MapXYZ), hipMemcpyHostToDevice));
     return VS_OK;
 }
 

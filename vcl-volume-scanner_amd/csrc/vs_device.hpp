@@ -119,8 +119,11 @@ struct alignas(16) SessParams {
     // buffers (device)
     const void *raw;         // W*H depth, level-0 size: uint16 mm, or float m when raw_f32
     float *filt[kMaxLevels];
-    MapXYZ *mvtx[kMaxLevels];  // model vertex, -100 invalid
-    MapXYZ *mnrm[kMaxLevels];  // model normal, -100 invalid
+    // Model vertex and normal maps, interleaved: pixel i's vertex is record
+    // 2i, its normal record 2i + 1 (-100 invalid), so the ICP's bilinear
+    // taps of two neighbouring pixels are 48 contiguous bytes; one padding
+    // pixel after the last keeps such a pair inside the buffer.
+    MapXYZ *mvn[kMaxLevels];
     uint8_t *mimg[kMaxLevels];
     void *tsdf;              // brick-major values
     void *weight;            // brick-major weights
@@ -236,6 +239,13 @@ __device__ __forceinline__ void stg3(MapXYZ *p, size_t i, float x, float y, floa
     q->x = x;
     q->y = y;
     q->z = z;
+}
+// pixel i of a level's interleaved model maps
+__device__ __forceinline__ MapXYZ ld_mvtx(const MapXYZ *mvn, size_t i) { return ldg3(mvn, 2 * i); }
+__device__ __forceinline__ MapXYZ ld_mnrm(const MapXYZ *mvn, size_t i) { return ldg3(mvn, 2 * i + 1); }
+__device__ __forceinline__ void st_mvtx(MapXYZ *mvn, size_t i, float x, float y, float z) { stg3(mvn, 2 * i, x, y, z); }
+__device__ __forceinline__ void st_mnrm(MapXYZ *mvn, size_t i, float x, float y, float z) {
+    stg3(mvn, 2 * i + 1, x, y, z);
 }
 
 // The top 3 rows of a row-major 4x4 FP32 matrix in HBM (16-byte aligned),

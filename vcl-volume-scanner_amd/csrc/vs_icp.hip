@@ -135,14 +135,31 @@ __device__ void publish_T21(SessState *st) {
 // ---------------------------------------------------------------------------
 // The 2x2 neighbourhood of (u, v) in a model map. A tap takes part only when
 // its weight is positive (rx > 0 for the right column, ry > 0 for the lower
-// row); a participating tap at -100 rejects the pixel. The four addresses are
-// always inside the map (an unused tap re-reads the anchor), so the taps of
-// both maps are loaded together, before any test.
+// row); a participating tap at -100 rejects the pixel. The maps interleave
+// vertex and normal records, so a row's two pixels x0, x0 + 1 are 48
+// contiguous bytes (vertex, normal, vertex, normal), read as three 16-byte
+// loads: an unused right column (rx == 0) is read and ignored (x0 + 1 <= W,
+// and the padding pixel after the last covers the last row), an unused
+// lower row re-reads the upper one. Every address is inside the buffer, so
+// all taps are loaded together, before any test.
 struct Quad {
     MapXYZ a, b, c, d;
 };
-__device__ __forceinline__ Quad load_quad(const MapXYZ *m, size_t i0, size_t di, size_t dj) {
-    return {ldg3(m, i0), ldg3(m, i0 + di), ldg3(m, i0 + dj), ldg3(m, i0 + di + dj)};
+typedef vs_f4 vs_f4_a4 __attribute__((aligned(4)));  // 24-byte records: 8-byte aligned
+struct Row {
+    vs_f4 q0, q1, q2;  // vertex x0 | normal x0 | vertex x0 + 1 | normal x0 + 1, 3 floats each
+};
+__device__ __forceinline__ Row load_row(const MapXYZ *mvn, size_t i) {
+    const VS_GLOBAL vs_f4_a4 *p = (const VS_GLOBAL vs_f4_a4 *)(mvn + 2 * i);
+    return {p[0], p[1], p[2]};
+}
+__device__ __forceinline__ Quad vertex_quad(const Row &r0, const Row &r1) {
+    return {{r0.q0.x, r0.q0.y, r0.q0.z}, {r0.q1.z, r0.q1.w, r0.q2.x}, {r1.q0.x, r1.q0.y, r1.q0.z},
+            {r1.q1.z, r1.q1.w, r1.q2.x}};
+}
+__device__ __forceinline__ Quad normal_quad(const Row &r0, const Row &r1) {
+    return {{r0.q0.w, r0.q1.x, r0.q1.y}, {r0.q2.y, r0.q2.z, r0.q2.w}, {r1.q0.w, r1.q1.x, r1.q1.y},
+            {r1.q2.y, r1.q2.z, r1.q2.w}};
 }
 // Taps are picked by selects, not branches, so that no tap's load is sunk
 // into a branch of its own (and waited for alone).
@@ -170,7 +187,7 @@ struct IcpPre {
     f3 p;            // the pixel's vertex in the model's frame
     float rx, ry;    // bilinear weights of the taps
     bool ok;         // depth valid, projection inside the map
-    Quad qv, qn;     // model vertex and normal taps (loads in flight)
+    Row r0, r1;      // model vertex and normal taps of the two rows (loads in flight)
 };
 // td1: the filtered depth at (x, y), loaded by the caller
 __device__ __forceinline__ IcpPre icp_pre(const SessParams &P, int l, int x, int y, float td1, const float *T21) {
@@ -186,19 +203,19 @@ __device__ __forceinline__ IcpPre icp_pre(const SessParams &P, int l, int x, int
     const int x0 = (int)uc, y0 = (int)vc;
     r.rx = uc - (float)x0;
     r.ry = vc - (float)y0;
-    const size_t i0 = (size_t)y0 * W + x0, di = r.rx > 0.0f ? 1 : 0, dj = r.ry > 0.0f ? (size_t)W : 0;
-    r.qv = load_quad(P.mvtx[l], i0, di, dj);
-    r.qn = load_quad(P.mnrm[l], i0, di, dj);
+    const size_t i0 = (size_t)y0 * W + x0, dj = r.ry > 0.0f ? (size_t)W : 0;
+    r.r0 = load_row(P.mvn[l], i0);
+    r.r1 = load_row(P.mvn[l], i0 + dj);
     return r;
 }
 __device__ __forceinline__ bool icp_post(const SessParams &P, const IcpPre &r, float A[6], float &b) {
     const f3 p = r.p;
     bool ok = r.ok;
     f3 v0, n;
-    ok &= bilerp(r.qv, r.rx, r.ry, v0);
+    ok &= bilerp(vertex_quad(r.r0, r.r1), r.rx, r.ry, v0);
     const float dx = v0.x - p.x, dy = v0.y - p.y, dz = v0.z - p.z;
     ok &= !(dx * dx + dy * dy + dz * dz > P.icp_dist2);
-    ok &= bilerp(r.qn, r.rx, r.ry, n);
+    ok &= bilerp(normal_quad(r.r0, r.r1), r.rx, r.ry, n);
     n = normalised(n);
     A[0] = ok ? p.z * n.y + (-p.y) * n.z : 0.0f;
     A[1] = ok ? (-p.z) * n.x + p.x * n.z : 0.0f;

@@ -1060,10 +1060,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const bool ok = cast_ray<TV, TW, STATS>(V, C, Tc.m, (float)x, (float)y, p, rs);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
-        stg3(P.mvtx[level], t, p.x, p.y, p.z);  // world hit; the shading makes it the camera-frame vertex
+        st_mvtx(P.mvn[level], t, p.x, p.y, p.z);  // world hit; the shading makes it the camera-frame vertex
     } else {
-        stg3(P.mvtx[level], t, -100.0f, -100.0f, -100.0f);
-        stg3(P.mnrm[level], t, -100.0f, -100.0f, -100.0f);
+        st_mvtx(P.mvn[level], t, -100.0f, -100.0f, -100.0f);
+        st_mnrm(P.mvn[level], t, -100.0f, -100.0f, -100.0f);
         stg(P.mimg[level], t, (uint8_t)0);
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
@@ -1099,7 +1099,7 @@ void k_raycast_shade(
     const size_t t = (size_t)y * L.W + x;
     // the march's output (it always runs just before): a world hit point,
     // or -100 for a miss, which is final already
-    const MapXYZ h = ldg3(P.mvtx[level], t);
+    const MapXYZ h = ld_mvtx(P.mvn[level], t);
     if (h.x == -100.0f) return;
     const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, nullptr};
     const f3 p = {h.x, h.y, h.z};
@@ -1115,8 +1115,8 @@ void k_raycast_shade(
         const f3 pc = transform(T, p);
         const f3 nc = {T[0] * n.x + T[1] * n.y + T[2] * n.z, T[4] * n.x + T[5] * n.y + T[6] * n.z,
                        T[8] * n.x + T[9] * n.y + T[10] * n.z};
-        stg3(P.mvtx[level], t, pc.x, pc.y, pc.z);
-        stg3(P.mnrm[level], t, nc.x, nc.y, nc.z);
+        st_mvtx(P.mvn[level], t, pc.x, pc.y, pc.z);
+        st_mnrm(P.mvn[level], t, nc.x, nc.y, nc.z);
         // light (0,0,-1): n.l = 0*nx + 0*ny - nz (z_yooji_object_scanner.cpp:1313)
         const float sa = nc.x * 0.0f + nc.y * 0.0f + nc.z * -1.0f;
         stg(P.mimg[level], t, (uint8_t)(fmaxf(0.0f, fminf(255.0f, (0.8f * sa + 0.2f) * 255.0f))));
@@ -1127,8 +1127,8 @@ void k_raycast_shade(
         }
         if (STATS) atomicAdd(stats + 2, 1ull);
     } else {
-        stg3(P.mvtx[level], t, -100.0f, -100.0f, -100.0f);
-        stg3(P.mnrm[level], t, -100.0f, -100.0f, -100.0f);
+        st_mvtx(P.mvn[level], t, -100.0f, -100.0f, -100.0f);
+        st_mnrm(P.mvn[level], t, -100.0f, -100.0f, -100.0f);
         stg(P.mimg[level], t, (uint8_t)0);
         if (P.mtex[level]) write_texture(P, level, t, 0u);
     }
@@ -1474,7 +1474,7 @@ __global__ __launch_bounds__(256) void k_maps_out(const SessParams *__restrict__
     const SessParams &P = SP[0];
     const size_t n = (size_t)P.lev[level].W * P.lev[level].H;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const MapXYZ v = ldg3(P.mvtx[level], i);
+        const MapXYZ v = ld_mvtx(P.mvn[level], i);
         if (depth) stg(depth, i, v.x == -100.0f ? 0.0f : v.z);
         if (vtx) {
             stg(vtx, 3 * i, v.x);
@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(256) void k_maps_out(const SessParams *__restrict__
             stg(vtx, 3 * i + 2, v.z);
         }
         if (nrm) {
-            const MapXYZ q = ldg3(P.mnrm[level], i);
+            const MapXYZ q = ld_mnrm(P.mvn[level], i);
             stg(nrm, 3 * i, q.x);
             stg(nrm, 3 * i + 1, q.y);
             stg(nrm, 3 * i + 2, q.z);
