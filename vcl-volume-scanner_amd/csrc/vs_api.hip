@@ -1063,12 +1063,21 @@ vs_status create_session(vs_ctx *ctx, const vs_session_desc *desc, const double 
     // surface reaches it, and the reference loop would not end for such a
     // ray (tsdf_renderer.cu:544-560 sums signed step lengths)
     P.max_iter = (int)(P.total_max / std::min(P.step_scale, 1.0f)) + 64;
-    // bilateral spatial weights exactly as updateGaussian (img_proc.cu:275-286)
-    for (int i = 0; i < 2 * d.bil_radius + 1; ++i) {
-        float x = (float)(i - d.bil_radius);
-        P.bil_g[i] = expf(-(x * x) / (2 * d.bil_sigma_s * d.bil_sigma_s));
+    // bilateral spatial weights exactly as updateGaussian (img_proc.cu:275-286),
+    // and their products g[i] g[j] as d_bilateral_filter forms them (:126-134)
+    {
+        const int n = 2 * d.bil_radius + 1;
+        float g[2 * 3 + 1];
+        for (int i = 0; i < n; ++i) {
+            float x = (float)(i - d.bil_radius);
+            g[i] = expf(-(x * x) / (2 * d.bil_sigma_s * d.bil_sigma_s));
+        }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) P.bil_w[i * n + j] = g[i] * g[j];
     }
-    P.bil_inv2s2 = 1.0f / (2.0f * d.bil_sigma_r * d.bil_sigma_r);
+    // exp(-x / (2 sigma_r^2)) = 2^(x * -log2(e) / (2 sigma_r^2)): one scaling
+    // on the host in place of two multiplications per tap
+    P.bil_nk = (float)(-1.4426950408889634 / (2.0 * (double)d.bil_sigma_r * (double)d.bil_sigma_r));
     P.bil_radius = d.bil_radius;
     P.raw = m + L.raw;
     for (int l = 0; l < kMaxLevels; ++l) {

@@ -113,8 +113,8 @@ struct alignas(16) SessParams {
     float cube_cen[3];
     float r_cube, step_scale, total_max;
     int max_iter;
-    float bil_g[8];     // 1-D spatial weights
-    float bil_inv2s2;   // 1 / (2 sigma_r^2)
+    float bil_w[49];    // spatial weights g[i] * g[j] (the reference's FP32 product), (2r+1)^2 row-major
+    float bil_nk;       // -log2(e) / (2 sigma_r^2): the range weight is 2^(bil_nk (d - c)^2)
     int bil_radius;
     // buffers (device)
     const void *raw;         // W*H depth, level-0 size: uint16 mm, or float m when raw_f32

@@ -15,20 +15,21 @@ constexpr int kHalo = 3;  // largest bilateral radius supported
 // The 2R+1 x 2R+1 window of one pixel (tile coordinates of the centre): sum
 // of w and of w (d - c), and whether a tap is <= 0. The radius is a
 // compile-time constant (1, 2 or 3, the radii a session accepts), so the
-// loops unroll.
+// loops unroll. The spatial products g[i] g[j] come from the session
+// parameters (uniform, in scalar registers) and the range weight is one
+// hardware base-2 exponential of the pre-scaled squared difference.
 template <int R, int TW>
-__device__ __forceinline__ void bil_window(const float (*tile)[TW], int cy, int cx, const float *g, float c,
-                                           float inv2s2, float &s, float &t, bool &zero) {
-    constexpr int r = R;
+__device__ __forceinline__ void bil_window(const float (*tile)[TW], int cy, int cx, const float *wij, float c,
+                                           float nk, float &s, float &t, bool &zero) {
+    constexpr int r = R, n = 2 * R + 1;
 #pragma unroll
     for (int i = -r; i <= r; ++i) {
-        const float gi = g[i + r];
 #pragma unroll
         for (int j = -r; j <= r; ++j) {
             const float p = tile[cy + i][cx + j];
             zero |= (p <= 0.0f);
             const float d = p - c;
-            const float w = gi * g[j + r] * __expf(-(d * d) * inv2s2);
+            const float w = wij[(i + r) * n + (j + r)] * __builtin_amdgcn_exp2f((d * d) * nk);
             t = fmaf(w, d, t);
             s += w;
         }
@@ -65,19 +66,24 @@ __global__ __launch_bounds__(kTileThreads) void k_bilateral(const SessParams *__
     const float c = tile[ly + kHalo][lx + kHalo];
     float s = 0.0f, t = 0.0f;
     bool zero = false;
-    float g[2 * kHalo + 1];
-#pragma unroll
-    for (int k = 0; k < 2 * kHalo + 1; ++k) g[k] = P.bil_g[k];
-    const float inv2s2 = P.bil_inv2s2;
+    const float nk = P.bil_nk;
     const int cy = ly + kHalo, cx = lx + kHalo;
-    // session creation accepts radii 1..3 only
-    if (R == 2)
-        bil_window<2, TW>(tile, cy, cx, g, c, inv2s2, s, t, zero);
-    else if (R == 1)
-        bil_window<1, TW>(tile, cy, cx, g, c, inv2s2, s, t, zero);
-    else
-        bil_window<3, TW>(tile, cy, cx, g, c, inv2s2, s, t, zero);
-    stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
+    // session creation accepts radii 1..3 only; each radius has its own
+    // complete path to the store
+    switch (R) {
+    case 1:
+        bil_window<1, TW>(tile, cy, cx, P.bil_w, c, nk, s, t, zero);
+        stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
+        break;
+    case 2:
+        bil_window<2, TW>(tile, cy, cx, P.bil_w, c, nk, s, t, zero);
+        stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
+        break;
+    case 3:
+        bil_window<3, TW>(tile, cy, cx, P.bil_w, c, nk, s, t, zero);
+        stg(P.filt[0], (size_t)y * W + x, zero ? 0.0f : c + t / s);
+        break;
+    }
 }
 
 // One pyramid step, level l-1 -> l: the separable [1 4 6 4 1]^2/256 blur of
