@@ -81,9 +81,6 @@ struct alignas(16) SessState {
     int force_single;  // T_init given: only level 0
     unsigned vis_count;         // visible bricks of the last integrate
     unsigned long long upd_count;  // voxels updated by the last integrate
-    double last_ATA[36];        // normal equations of the last solve (diagnostic)
-    double last_ATb[6];
-    double last_valid;
 };
 
 // One element of a model vertex or normal map: x, y, z (12 bytes, AoS like

@@ -87,24 +87,25 @@ __device__ void expmap(const double *x, double *T) {
 // before it (a plane leaves 3 of the 6 motions unconstrained), a near-rank
 // deficiency the absolute test cannot see at FP32 scales. The ratio is
 // invariant to the units of the rotation and translation columns.
+// Column by column, each below-diagonal entry and substitution step scaled by
+// the column's reciprocal pivot (6 FP64 divisions instead of 33 on the
+// solve's one-lane chain; the same sums in the same order).
 // Returns 0 on success.
 __device__ int cholesky_solve(const double *A, const double *b, double rel, double *x) {
-    double L[36];
+    double L[36], inv[6];
     for (int k = 0; k < 36; ++k) L[k] = 0.0;
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j <= i; ++j) {
-            if (i == j) {
-                double s = 0.0;
-                for (int k = 0; k < j; ++k) s += L[j * 6 + k] * L[j * 6 + k];
-                L[j * 6 + j] = sqrt(A[j * 6 + j] - s);
-            } else if (fabs(L[j * 6 + j]) >= 1e-8) {
-                double s = 0.0;
-                for (int k = 0; k < j; ++k) s += L[i * 6 + k] * L[j * 6 + k];
-                L[i * 6 + j] = (A[i * 6 + j] - s) / L[j * 6 + j];
-            } else {
-                L[i * 6 + j] = 0.0;
-            }
+    for (int j = 0; j < 6; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < j; ++k) s += L[j * 6 + k] * L[j * 6 + k];
+        L[j * 6 + j] = sqrt(A[j * 6 + j] - s);
+        inv[j] = 1.0 / L[j * 6 + j];
+        const bool piv = fabs(L[j * 6 + j]) >= 1e-8;
+        for (int i = j + 1; i < 6; ++i) {
+            double t = 0.0;
+            for (int k = 0; k < j; ++k) t += L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = piv ? (A[i * 6 + j] - t) * inv[j] : 0.0;
         }
+    }
     for (int i = 0; i < 6; ++i)
         if (!(fabs(L[i * 6 + i]) >= 1e-8)) return 4;
     if (rel > 0.0)
@@ -114,12 +115,12 @@ __device__ int cholesky_solve(const double *A, const double *b, double rel, doub
     for (int i = 0; i < 6; ++i) {
         double s = 0.0;
         for (int k = 0; k < i; ++k) s += L[i * 6 + k] * y[k];
-        y[i] = (b[i] - s) / L[i * 6 + i];
+        y[i] = (b[i] - s) * inv[i];
     }
     for (int i = 5; i >= 0; --i) {
         double s = 0.0;
         for (int k = i + 1; k < 6; ++k) s += L[k * 6 + i] * x[k];
-        x[i] = (y[i] - s) / L[i * 6 + i];
+        x[i] = (y[i] - s) * inv[i];
     }
     return 0;
 }
@@ -437,12 +438,20 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_iter(const SessParams *__r
 #endif
 }
 
-// Second half: one workgroup per session sums the group partials in FP64 in
-// a fixed order (thread t adds groups t, t + 256, ..., then a fixed tree over
-// the threads; the result does not depend on the batch or the schedule),
-// solves the 6x6 system and applies the SE(3) update.
-__global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__restrict__ SP, int nsess, int level,
-                                                            int iter) {
+// Second half: one wave per session sums the group partials in FP64 in a
+// fixed order, solves the 6x6 system and applies the SE(3) update. Lane t
+// adds groups t, t + 64, ...; then the 29 sums (padded to 32) are reduced
+// across the wave by recursive halving: at each of five exchanges a lane
+// sends half of the sums it still holds to its partner (lane ^ 32, ^ 16, ...,
+// ^ 2) and adds the partner's half of its own, so 16 + 8 + 4 + 2 + 1 values
+// cross lanes instead of 29 x 6 for 29 separate butterflies; a last exchange
+// with lane ^ 1 completes sum v = lane / 2, and lane 0 reads the 29 totals
+// with readlane. The order is fixed, so the result does not depend on the
+// batch or the schedule. (A 256-thread block with a butterfly per sum spent
+// about twice as long in this part.)
+constexpr int kSolveThreads = 64;
+__global__ __launch_bounds__(kSolveThreads) void k_icp_solve(const SessParams *__restrict__ SP, int nsess, int level,
+                                                             int iter) {
     const int s = blockIdx.x;
     if (s >= nsess) return;
     const SessParams &P = SP[s];
@@ -450,27 +459,34 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
     if (level >= P.levels || icp_skip(P, st, level, iter)) return;
     const LevelDesc &L = P.lev[level];
     const int ngroups = icp_groups(L.tiles_x * L.tiles_y, level);
-    double acc[kIcpVals];
+    const int lane = threadIdx.x;
+    double acc[32];
 #pragma unroll
-    for (int k = 0; k < kIcpVals; ++k) acc[k] = 0.0;
-    for (int t = threadIdx.x; t < ngroups; t += kTileThreads) {
+    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+    for (int t = lane; t < ngroups; t += kSolveThreads) {
         const double *pp = P.icp_part + (size_t)t * kIcpVals;
 #pragma unroll
         for (int k = 0; k < kIcpVals; ++k) acc[k] += ldg(pp, k);
     }
-    __shared__ double wred[kTileThreads / 64][kIcpVals];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < kIcpVals; ++k) {
-        double s = acc[k];
+    for (int h = 16; h >= 1; h >>= 1) {  // partner lane ^ 2h; keep the upper h sums when that bit is set
+        const bool up = (lane & (2 * h)) != 0;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) wred[wv][k] = s;
+        for (int i = 0; i < h; ++i) {
+            const double send = up ? acc[i] : acc[h + i];
+            const double keep = up ? acc[h + i] : acc[i];
+            acc[i] = keep + __shfl_xor(send, 2 * h, 64);
+        }
     }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+    const double own = acc[0] + __shfl_xor(acc[0], 1, 64);  // sum (lane / 2) over all 64 lanes
     double tot[kIcpVals];
-    for (int k = 0; k < kIcpVals; ++k) tot[k] = wred[0][k] + wred[1][k] + wred[2][k] + wred[3][k];
+    {
+        const int lo = __double2loint(own), hi = __double2hiint(own);
+#pragma unroll
+        for (int k = 0; k < kIcpVals; ++k)
+            tot[k] = __hiloint2double(__builtin_amdgcn_readlane(hi, 2 * k), __builtin_amdgcn_readlane(lo, 2 * k));
+    }
+    if (lane != 0) return;
     double ATA[36], ATb[6], x6[6];
     int k = 0;
     for (int i = 0; i < 6; ++i)
@@ -483,9 +499,6 @@ __global__ __launch_bounds__(kTileThreads) void k_icp_solve(const SessParams *__
     // in one thread waits on both the vector-memory and the scalar counters)
     double T12[16];
     for (int i = 0; i < 16; ++i) T12[i] = ldg(st->T_12, i);
-    for (int i = 0; i < 36; ++i) stg(st->last_ATA, i, ATA[i]);
-    for (int i = 0; i < 6; ++i) stg(st->last_ATb, i, ATb[i]);
-    stg(&st->last_valid, 0, cnt);
     stg(&st->solves, 0, ldg(&st->solves, 0) + 1);
     if (P.icp_min_valid > 0 && cnt < (double)P.icp_min_valid) {
         stg(&st->icp_failed, 0, 7);
@@ -685,7 +698,7 @@ void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess
     case 4: hipLaunchKernelGGL(k_icp_iter<4>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
     default: hipLaunchKernelGGL(k_icp_iter<8>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
     }
-    hipLaunchKernelGGL(k_icp_solve, dim3(nsess), dim3(kTileThreads), 0, s, P, nsess, level, iter);
+    hipLaunchKernelGGL(k_icp_solve, dim3(nsess), dim3(kSolveThreads), 0, s, P, nsess, level, iter);
 }
 void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s) {
     hipLaunchKernelGGL(k_track_finalize, dim3((nsess + 63) / 64), dim3(64), 0, s, P, nsess);
