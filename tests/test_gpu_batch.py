@@ -141,3 +141,29 @@ def test_failure_keeps_pose_and_skips_fusion(ctx):
         assert s.status()[0] == 0
     finally:
         s.close()
+
+
+def test_wide_offsets_equal_32bit_offsets(ctx):
+    """The shading addresses the volume with 32-bit offsets when every
+    session of a launch has value and weight arrays below 4 GB, and with
+    64-bit addresses otherwise. A session batched with a 1024^3 FP32 session
+    (4 GB of values) takes the 64-bit path and must give the same bits as
+    the same session alone (32-bit path)."""
+    print("case: 128^3 session alone vs batched with a 1024^3 FP32 session, 4 frames", flush=True)
+    (st, d), (stb, db) = _sessions(ctx, vscan.VS_TSDF_F32, vscan.VS_WEIGHT_F32,
+                                   [(4, 1, 320, 240, 128), (3, 7, 320, 240, 1024)])
+    assert vscan.session_bytes(db) > 8 << 30  # 4 GiB of values and 4 GiB of weights
+    alone = vscan.Session(ctx, d, T_gc0=st.pose(0))
+    mixed = vscan.Session(ctx, d, T_gc0=st.pose(0))
+    big = vscan.Session(ctx, db, T_gc0=stb.pose(0))
+    try:
+        for fr in range(4):
+            alone.fuse(st.frame(fr))
+            ctx.fuse_batch([mixed, big], [st.frame(fr), stb.frame(fr)])
+            assert alone.status() == mixed.status()
+            assert np.array_equal(alone.pose(), mixed.pose()), f"frame {fr}: pose differs"
+        for l in range(3):
+            assert all(np.array_equal(x, y) for x, y in zip(alone.maps(l), mixed.maps(l))), f"maps L{l} differ"
+    finally:
+        for s in (alone, mixed, big):
+            s.close()

@@ -266,6 +266,7 @@ struct Batch {
     BrickList bl{};
     uint32_t bricks = 0;    // capacity of the brick list
     uint32_t max_words = 0; // longest visible-brick bitmap of a member (32-bit words)
+    bool o32 = false;       // every member's value and weight arrays are below 4 GB
     int integ_grid = 1;     // default integrate grid
     Shape shape;
     std::string size_class;
@@ -365,11 +366,14 @@ namespace {
 struct ListSet {
     std::vector<WorkItem> tile[kMaxLevels], group[kMaxLevels];
     uint32_t bricks = 0, max_words = 0;
+    size_t max_array = 0;  // largest value or weight array of a member (bytes)
     int levels = 0, max_iters[kMaxLevels] = {0, 0, 0};
     int count = 0, max_w = 0, max_h = 0, max_dim = 0;
     void add(int slot, const vs_session *s) {
         ++count;
         max_words = std::max(max_words, s->L.nwords);
+        max_array = std::max(max_array, s->L.nvox * std::max(s->d.tsdf_type == VS_TSDF_F16 ? 2 : 4,
+                                                             s->d.weight_type == VS_WEIGHT_U8 ? 1 : 4));
         max_w = std::max(max_w, s->d.width);
         max_h = std::max(max_h, s->d.height);
         for (int k = 0; k < 3; ++k) max_dim = std::max(max_dim, s->d.vol_dim[k]);
@@ -455,6 +459,7 @@ struct ListSet {
         for (int l = 0; l < kMaxLevels; ++l) B.max_iters[l] = max_iters[l];
         B.bricks = bricks;
         B.max_words = max_words;
+        B.o32 = max_array < (size_t(1) << 32);
         B.integ_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(bricks, (uint32_t)integ_max));
         // size class: largest frame, largest volume edge and session count
         // rounded up to powers of two, storage types
@@ -694,7 +699,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, s);
     mark(6);
     launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, kShadeOpts[B.shape.shade].block,
-                         kShadeOpts[B.shape.shade].waves, s);
+                         kShadeOpts[B.shape.shade].waves, B.o32, s);
     mark(7);
     mark(8);
     if (timed) timed->used = true;
@@ -1388,7 +1393,7 @@ vs_status vs_raycast(vs_session *s, int level, vs_stream stream) {
     hipStream_t h = pick(s, stream);
     const Batch &B = s->self;
     launch_fuse_ok(B.dP, 1, h);
-    launch_raycast(B.dP, B.dWork[level], B.nWork[level], level, B.tt, B.wt, h);
+    launch_raycast(B.dP, B.dWork[level], B.nWork[level], level, B.tt, B.wt, B.o32, h);
     HIP_TRY(hipGetLastError());
     return VS_OK;
 }

@@ -298,6 +298,14 @@ __device__ __forceinline__ size_t vox_index(const SessParams &P, int x, int y, i
     size_t b = ((size_t)(z >> 3) * P.bdim[1] + (y >> 3)) * P.bdim[0] + (x >> 3);
     return b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
 }
+// The same in 32 bits, for value and weight arrays below 4 GB (the caller
+// checks): the loads then take a 32-bit byte offset from the array base held
+// in scalar registers instead of a 64-bit address per lane.
+__device__ __forceinline__ uint32_t vox_index32(const SessParams &P, int x, int y, int z) {
+    const uint32_t b = ((uint32_t)(z >> 3) * (uint32_t)P.bdim[1] + (uint32_t)(y >> 3)) * (uint32_t)P.bdim[0] +
+                       (uint32_t)(x >> 3);
+    return b * kBrickVox + (uint32_t)(((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
+}
 __device__ __forceinline__ int brick_of(const SessParams &P, int x, int y, int z) {
     return ((z >> 3) * P.bdim[1] + (y >> 3)) * P.bdim[0] + (x >> 3);
 }
@@ -317,6 +325,16 @@ __device__ __forceinline__ void st_val(__half *p, size_t i, float v) {
 }
 __device__ __forceinline__ float ld_w(const float *p, size_t i) { return ldg(p, i); }
 __device__ __forceinline__ float ld_w(const uint8_t *p, size_t i) { return (float)ldg(p, i); }
+template <class T>
+__device__ __forceinline__ T ldg_off(const T *p, uint32_t byte_off) {
+    return *(const VS_GLOBAL T *)((const VS_GLOBAL char *)p + byte_off);
+}
+__device__ __forceinline__ float ld_val32(const float *p, uint32_t i) { return ldg_off(p, i << 2); }
+__device__ __forceinline__ float ld_val32(const __half *p, uint32_t i) {
+    return __half2float(__ushort_as_half(ldg_off((const uint16_t *)p, i << 1)));
+}
+__device__ __forceinline__ float ld_w32(const float *p, uint32_t i) { return ldg_off(p, i << 2); }
+__device__ __forceinline__ float ld_w32(const uint8_t *p, uint32_t i) { return (float)ldg_off(p, i); }
 __device__ __forceinline__ void st_w(float *p, size_t i, float v) { stg(p, i, v); }
 __device__ __forceinline__ void st_w(uint8_t *p, size_t i, float v) { stg(p, i, (uint8_t)v); }
 

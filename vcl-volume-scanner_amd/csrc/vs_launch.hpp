@@ -52,8 +52,10 @@ void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, int nsess
 // sd_slices: the most superbricks along z of any session (distance field)
 void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_type, int weight_type, int grid,
                       int sd_slices, hipStream_t s);
+// o32: every session's value and weight arrays are below 4 GB (the shading
+// then addresses them with 32-bit offsets)
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
-                    hipStream_t s);
+                    bool o32, hipStream_t s);
 // a single session's model maps of `level` (npx pixels) into caller buffers
 // (each nullable): depth, vertex and normal (3 floats per pixel), image
 void launch_maps_out(const SessParams *P, int level, size_t npx, float *depth, float *vtx, float *nrm, uint8_t *img,
@@ -79,9 +81,9 @@ void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipS
 enum RayBlock { kRay8x8 = 0, kRay16x4 = 1, kRay32x2 = 2 };
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
                           unsigned long long *stats, int ray_block, hipStream_t s);
-// waves: the waves per SIMD its registers aim for (4, 5 or 6)
+// waves: the waves per SIMD its registers aim for (4 or 5); o32 as for launch_raycast
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
-                          unsigned long long *stats, int ray_block, int waves, hipStream_t s);
+                          unsigned long long *stats, int ray_block, int waves, bool o32, hipStream_t s);
 // volume maintenance (single session, P points at its parameters)
 void launch_volume_init(const SessParams *P, size_t nvox, int tsdf_type, int weight_type, uint32_t nwords,
                         int sd_slices, hipStream_t s);

@@ -574,6 +574,17 @@ struct Vol {
         out = ok ? f : 1.0f;
         return ok;
     }
+    // voxel (x, y, z) in range; O32: 32-bit offsets (arrays below 4 GB)
+    template <bool O32>
+    __device__ __forceinline__ bool load_at(int x, int y, int z, float &out) const {
+        if (!O32) return load(vox_index(P, x, y, z), out);
+        const uint32_t idx = vox_index32(P, x, y, z);
+        const float w = ld_w32(Wt, idx);
+        const float f = ld_val32(F, idx);
+        const bool ok = w > 0.0f;
+        out = ok ? f : 1.0f;
+        return ok;
+    }
     // d_gtvu, exactly: nearest voxel by ROUNDF; true when it has weight.
     __device__ __forceinline__ bool nearest_valid(f3 v, float &out) const {
         int x, y, z;
@@ -880,7 +891,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
 
 // d_csnt_Compute_Surface_Normal_from_TSDF: central differences of trilinear
 // values from 32 taps around the hit.
-template <class TV, class TW>
+template <bool O32, class TV, class TW>
 __device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n) {
     f3 vox;
     if (!world_to_voxel(V.P, p, vox)) return false;
@@ -897,13 +908,13 @@ __device__ __forceinline__ bool surface_normal(const Vol<TV, TW> &V, f3 p, f3 &n
     float fm[4], fr[12], bk[12], bm[4];
     bool ok = true;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) ok &= V.load(vox_index(V.P, ox + dx1[i], oy + dy1[i], oz - 1), fm[i]);
+    for (int i = 0; i < 4; ++i) ok &= V.template load_at<O32>(ox + dx1[i], oy + dy1[i], oz - 1, fm[i]);
 #pragma unroll
-    for (int i = 0; i < 12; ++i) ok &= V.load(vox_index(V.P, ox + dx2[i], oy + dy2[i], oz), fr[i]);
+    for (int i = 0; i < 12; ++i) ok &= V.template load_at<O32>(ox + dx2[i], oy + dy2[i], oz, fr[i]);
 #pragma unroll
-    for (int i = 0; i < 12; ++i) ok &= V.load(vox_index(V.P, ox + dx2[i], oy + dy2[i], oz + 1), bk[i]);
+    for (int i = 0; i < 12; ++i) ok &= V.template load_at<O32>(ox + dx2[i], oy + dy2[i], oz + 1, bk[i]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) ok &= V.load(vox_index(V.P, ox + dx1[i], oy + dy1[i], oz + 2), bm[i]);
+    for (int i = 0; i < 4; ++i) ok &= V.template load_at<O32>(ox + dx1[i], oy + dy1[i], oz + 2, bm[i]);
     if (!ok) return false;
     auto Lz = [&](int i) { return (1.0f - rz) * fr[i] + rz * bk[i]; };
     float a4[4], b4[4], a2[2], b2[2];
@@ -1080,8 +1091,11 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
 // WV: the waves per SIMD the register allocation aims for (4: the compiler's
 // own choice, ~126 VGPRs with all 64 stencil loads in flight; 5: 91, fewer
 // loads in flight); the same results for each, so the autotuner picks. (6
-// waves, 80 VGPRs, spills a few registers and was slower still.)
-template <class TV, class TW, bool STATS, int RB, int WV>
+// waves, 80 VGPRs, spills a few registers and was slower still.) O32: the
+// stencil's loads take 32-bit offsets from the array bases (every session
+// of the launch has value and weight arrays below 4 GB): 80 VGPRs instead of
+// 126 at WV 4, shading 0.93 -> 0.89 ms; the same results.
+template <class TV, class TW, bool STATS, int RB, int WV, bool O32>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(WV == 4 ? 1 : WV, WV == 4 ? 8 : WV)))
 void k_raycast_shade(
     const SessParams *__restrict__ SP, LevelWork lw, unsigned long long *stats) {
@@ -1109,7 +1123,7 @@ void k_raycast_shade(
         atomicAdd(stats + 7, b);
     }
     f3 n;
-    if (surface_normal(V, p, n)) {
+    if (surface_normal<O32>(V, p, n)) {
         const M34 Tg = ld_m34(st->T_gc_f);
         const float *T = Tg.m;
         const f3 pc = transform(T, p);
@@ -1357,14 +1371,21 @@ static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long
 }
 template <class TV, class TW>
 static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, int waves,
-                    hipStream_t s) {
+                    bool o32, hipStream_t s) {
     const int n = lw.blocks();
     if (n <= 0) return;
-#define VS_SHADE(RB_, WV_) \
-    hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, RB_, WV_>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats)
+#define VS_SHADE(RB_, WV_)                                                                                      \
+    do {                                                                                                        \
+        if (o32)                                                                                                \
+            hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, RB_, WV_, true>), dim3(n), dim3(kTileThreads), 0, \
+                               s, P, lw, stats);                                                                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_raycast_shade<TV, TW, false, RB_, WV_, false>), dim3(n), dim3(kTileThreads),  \
+                               0, s, P, lw, stats);                                                             \
+    } while (0)
     if (stats)
-        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true, kRay32x2, 4>), dim3(n), dim3(kTileThreads), 0, s, P, lw,
-                           stats);
+        hipLaunchKernelGGL((k_raycast_shade<TV, TW, true, kRay32x2, 4, false>), dim3(n), dim3(kTileThreads), 0, s, P,
+                           lw, stats);
     else if (rb == kRay8x8 && waves == 5)
         VS_SHADE(kRay8x8, 5);
     else if (rb == kRay8x8)
@@ -1391,8 +1412,8 @@ void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tt, int 
     VS_DISPATCH(tt, wt, march_t, P, lw, stats, rb, s);
 }
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
-                          int rb, int waves, hipStream_t s) {
-    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, rb, waves, s);
+                          int rb, int waves, bool o32, hipStream_t s) {
+    VS_DISPATCH(tt, wt, shade_t, P, lw, stats, rb, waves, o32, s);
 }
 static LevelWork one_level(const WorkItem *work, int n, int level) {
     LevelWork lw{};
@@ -1404,16 +1425,17 @@ static LevelWork one_level(const WorkItem *work, int n, int level) {
     }
     return lw;
 }
-void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, hipStream_t s) {
+void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, bool o32,
+                    hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
     launch_raycast_march(P, lw, tt, wt, nullptr, kRay8x8, s);
-    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay8x8, 4, s);
+    launch_raycast_shade(P, lw, tt, wt, nullptr, kRay8x8, 4, o32, s);
 }
 void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
                           unsigned long long *stats, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
     launch_raycast_march(P, lw, tt, wt, stats, kRay8x8, s);
-    launch_raycast_shade(P, lw, tt, wt, stats, kRay32x2, 4, s);
+    launch_raycast_shade(P, lw, tt, wt, stats, kRay32x2, 4, false, s);
 }
 
 static int grid_for(size_t n) {
