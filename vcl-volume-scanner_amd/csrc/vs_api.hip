@@ -216,8 +216,9 @@ struct Timer {
 // depends on the workload, so a context can pick them per size class of
 // batch (frame size, volume size, storage types, session count).
 // The shading's candidates: ray block and the waves per SIMD its register
-// allocation aims for (4: every stencil load in flight at ~126 VGPRs; 5: 91
-// VGPRs).
+// allocation aims for (4: the compiler's own choice, 80 VGPRs with 32-bit
+// stencil offsets or ~126 with 64-bit ones, every stencil load in flight; 5:
+// at most 102).
 struct ShadeOpt {
     int block, waves;
     const char *name;
