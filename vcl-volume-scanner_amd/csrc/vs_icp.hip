@@ -689,15 +689,14 @@ void launch_set_active(int *active, int nslots, const uint32_t bits[kMaxSlots / 
 void launch_icp_iter(const SessParams *P, const WorkItem *work, int n, int nsess, int level, int iter,
                      hipStream_t s) {
     if (n <= 0) return;
-    static_assert(icp_group(0) == 4 || icp_group(0) == 2 || icp_group(0) == 1 || icp_group(0) == 8, "group size");
-    static_assert(icp_group(1) == 4 || icp_group(1) == 2 || icp_group(1) == 1 || icp_group(1) == 8, "group size");
-    static_assert(icp_group(2) == 4 || icp_group(2) == 2 || icp_group(2) == 1 || icp_group(2) == 8, "group size");
-    switch (icp_group(level)) {
-    case 1: hipLaunchKernelGGL(k_icp_iter<1>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
-    case 2: hipLaunchKernelGGL(k_icp_iter<2>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
-    case 4: hipLaunchKernelGGL(k_icp_iter<4>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
-    default: hipLaunchKernelGGL(k_icp_iter<8>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter); break;
-    }
+    static_assert(kMaxLevels == 3, "one launch case per level");
+    // one instantiation per distinct group size in use
+    if (level == 0)
+        hipLaunchKernelGGL(k_icp_iter<icp_group(0)>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
+    else if (level == 1)
+        hipLaunchKernelGGL(k_icp_iter<icp_group(1)>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
+    else
+        hipLaunchKernelGGL(k_icp_iter<icp_group(2)>, dim3(n), dim3(kTileThreads), 0, s, P, work, level, iter);
     hipLaunchKernelGGL(k_icp_solve, dim3(nsess), dim3(kSolveThreads), 0, s, P, nsess, level, iter);
 }
 void launch_track_finalize(const SessParams *P, int nsess, hipStream_t s) {
