@@ -72,9 +72,10 @@ __global__ __launch_bounds__(kTileThreads) void k_vis_mark(const SessParams *__r
             f3 tv;
             if (!world_to_voxel(P, q, tv)) continue;
             // ROUNDF(v)/8 with the macro's precedence: the negative branch is not divided
-            const int bx = (tv.x < 0.0f) ? (int)(tv.x - 0.5f) : (int)(tv.x + 0.5f) / kBrick;
-            const int by = (tv.y < 0.0f) ? (int)(tv.y - 0.5f) : (int)(tv.y + 0.5f) / kBrick;
-            const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)(tv.z + 0.5f) / kBrick;
+            // (the positive branch's integer is >= 0, so / 8 is a shift)
+            const int bx = (tv.x < 0.0f) ? (int)(tv.x - 0.5f) : (int)((unsigned)(int)(tv.x + 0.5f) >> 3);
+            const int by = (tv.y < 0.0f) ? (int)(tv.y - 0.5f) : (int)((unsigned)(int)(tv.y + 0.5f) >> 3);
+            const int bz = (tv.z < 0.0f) ? (int)(tv.z - 0.5f) : (int)((unsigned)(int)(tv.z + 0.5f) >> 3);
             if (bx < 0 || bx >= P.bdim[0] || by < 0 || by >= P.bdim[1] || bz < 0 || bz >= P.bdim[2]) break;
             bb[n + 4] = ((uint32_t)bz * P.bdim[1] + by) * P.bdim[0] + bx;  // 0xFFFFFFFF: no brick
         }
