@@ -37,7 +37,7 @@ def test_session_bytes_and_descriptor_checks():
     w = vscan.WORKLOADS["fp16_640x480_512"]
     d = vscan.make_desc(w)
     nb = vscan.session_bytes(d)
-    vol = 512 ** 3 * 3  # FP16 value + u8 weight
+    vol = 512 ** 3 * 4  # FP16 value + u8 weight in one 4-byte record per voxel
     assert vol < nb < vol * 1.1
     d32 = vscan.make_desc(vscan.WORKLOADS["plain_640x480_256"])
     assert 256 ** 3 * 8 < vscan.session_bytes(d32) < 256 ** 3 * 8 * 1.2

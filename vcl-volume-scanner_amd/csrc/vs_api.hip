@@ -65,7 +65,7 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
 struct Layout {
     size_t state, raw, filt[kMaxLevels], mvn[kMaxLevels], mimg[kMaxLevels];
-    size_t tsdf, weight, bvis, vmask, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
+    size_t vox, bvis, vmask, bocc, bband, sband, sdist, sdtmp, icp_part, diag_part, diag_mask, lists, params, stats, keys;
     size_t vcol, rgb, mtex[kMaxLevels];  // colour sessions only (0 otherwise)
     size_t wtab;                          // voxel-centre world coordinates, x then y then z
     size_t total;
@@ -117,8 +117,9 @@ Layout make_layout(const vs_session_desc &d) {
         L.mvn[l] = take((n + 1) * 2 * sizeof(MapXYZ));  // vertex + normal per pixel, one padding pixel
         L.mimg[l] = take(n);
     }
-    L.tsdf = take(L.nvox * sv);
-    L.weight = take(L.nvox * sw);
+    // value and weight of a voxel in one record (4 bytes for FP16 + u8, 8
+    // otherwise)
+    L.vox = take(L.nvox * vox_rec_bytes(sv == 2, sw == 1));
     L.bvis = take((size_t)L.nwords * 4);
     L.vmask = take(((size_t)L.nbricks + 127) / 128 * 128);
     L.bocc = take((size_t)L.nwords * 4);
@@ -267,7 +268,7 @@ struct Batch {
     BrickList bl{};
     uint32_t bricks = 0;    // capacity of the brick list
     uint32_t max_words = 0; // longest visible-brick bitmap of a member (32-bit words)
-    bool o32 = false;       // every member's value and weight arrays are below 4 GB
+    bool o32 = false;       // every member's voxel records are below 4 GB
     int integ_grid = 1;     // default integrate grid
     Shape shape;
     std::string size_class;
@@ -367,14 +368,14 @@ namespace {
 struct ListSet {
     std::vector<WorkItem> tile[kMaxLevels], group[kMaxLevels];
     uint32_t bricks = 0, max_words = 0;
-    size_t max_array = 0;  // largest value or weight array of a member (bytes)
+    size_t max_array = 0;  // largest voxel-record array of a member (bytes)
     int levels = 0, max_iters[kMaxLevels] = {0, 0, 0};
     int count = 0, max_w = 0, max_h = 0, max_dim = 0;
     void add(int slot, const vs_session *s) {
         ++count;
         max_words = std::max(max_words, s->L.nwords);
-        max_array = std::max(max_array, s->L.nvox * std::max(s->d.tsdf_type == VS_TSDF_F16 ? 2 : 4,
-                                                             s->d.weight_type == VS_WEIGHT_U8 ? 1 : 4));
+        max_array = std::max(max_array, s->L.nvox * vox_rec_bytes(s->d.tsdf_type == VS_TSDF_F16,
+                                                                  s->d.weight_type == VS_WEIGHT_U8));
         max_w = std::max(max_w, s->d.width);
         max_h = std::max(max_h, s->d.height);
         for (int k = 0; k < 3; ++k) max_dim = std::max(max_dim, s->d.vol_dim[k]);
@@ -1091,8 +1092,7 @@ vs_status create_session(vs_ctx *ctx, const vs_session_desc *desc, const double 
         P.mvn[l] = (MapXYZ *)(m + L.mvn[l]);
         P.mimg[l] = (uint8_t *)(m + L.mimg[l]);
     }
-    P.tsdf = m + L.tsdf;
-    P.weight = m + L.weight;
+    P.vox = m + L.vox;
     P.bvis = (uint32_t *)(m + L.bvis);
     P.vmask = (uint8_t *)(m + L.vmask);
     P.bocc = (uint32_t *)(m + L.bocc);

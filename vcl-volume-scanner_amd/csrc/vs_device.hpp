@@ -122,8 +122,7 @@ struct alignas(16) SessParams {
     // pixel after the last keeps such a pair inside the buffer.
     MapXYZ *mvn[kMaxLevels];
     uint8_t *mimg[kMaxLevels];
-    void *tsdf;              // brick-major values
-    void *weight;            // brick-major weights
+    void *vox;               // brick-major voxel records (value and weight, VoxRec)
     uint32_t *bvis;          // visible-brick bits of the current frame
     uint8_t *vmask;          // visible-brick flags of the current frame, one byte per brick
                              // (zero-padded to a multiple of 128 bricks)
@@ -298,7 +297,7 @@ __device__ __forceinline__ size_t vox_index(const SessParams &P, int x, int y, i
     size_t b = ((size_t)(z >> 3) * P.bdim[1] + (y >> 3)) * P.bdim[0] + (x >> 3);
     return b * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
 }
-// The same in 32 bits, for value and weight arrays below 4 GB (the caller
+// The same in 32 bits, for voxel-record arrays below 4 GB (the caller
 // checks): the loads then take a 32-bit byte offset from the array base held
 // in scalar registers instead of a 64-bit address per lane.
 __device__ __forceinline__ uint32_t vox_index32(const SessParams &P, int x, int y, int z) {
@@ -314,28 +313,76 @@ __device__ __forceinline__ int super_of_brick(const SessParams &P, int bx, int b
     return ((bz >> kSuperShift) * P.sdim[1] + (by >> kSuperShift)) * P.sdim[0] + (bx >> kSuperShift);
 }
 
-// Storage conversions for the two volume forms (volume planes are in HBM).
-__device__ __forceinline__ float ld_val(const float *p, size_t i) { return ldg(p, i); }
-__device__ __forceinline__ float ld_val(const __half *p, size_t i) {
-    return __half2float(__ushort_as_half(ldg((const uint16_t *)p, i)));
+// Voxel records. A voxel's value and weight share one record, so a lookup
+// is one load and one address: 4 bytes for FP16 value + u8 weight (value in
+// bits 0-15, weight in bits 16-23, bits 24-31 zero), 8 bytes for the other
+// forms (FP32 value in word 0 or FP16 value in bits 0-15 of word 0; FP32
+// weight in word 1 or u8 weight in bits 0-7 of word 1). The conversions are
+// the ones separate planes used: FP16 values round to nearest, u8 weights
+// are the truncated float.
+typedef unsigned int vs_u2 __attribute__((ext_vector_type(2)));
+template <class TV, class TW>
+struct VoxRec {
+    typedef vs_u2 T;  // 8-byte forms
+    static __device__ __forceinline__ float val(T r) {
+        if constexpr (sizeof(TV) == 4) return __uint_as_float(r.x);
+        else return __half2float(__ushort_as_half((unsigned short)(r.x & 0xFFFFu)));
+    }
+    static __device__ __forceinline__ float w(T r) {
+        if constexpr (sizeof(TW) == 4) return __uint_as_float(r.y);
+        else return (float)(r.y & 0xFFu);
+    }
+    static __device__ __forceinline__ bool has_w(T r) {
+        if constexpr (sizeof(TW) == 4) return __uint_as_float(r.y) > 0.0f;
+        else return (r.y & 0xFFu) != 0u;
+    }
+    static __device__ __forceinline__ T make(float v, float wt) {
+        T r;
+        if constexpr (sizeof(TV) == 4) r.x = __float_as_uint(v);
+        else r.x = (unsigned)__half_as_ushort(__float2half_rn(v));
+        if constexpr (sizeof(TW) == 4) r.y = __float_as_uint(wt);
+        else r.y = (unsigned)(uint8_t)wt;
+        return r;
+    }
+};
+template <>
+struct VoxRec<__half, uint8_t> {
+    typedef uint32_t T;
+    static __device__ __forceinline__ float val(T r) { return __half2float(__ushort_as_half((unsigned short)(r & 0xFFFFu))); }
+    static __device__ __forceinline__ float w(T r) { return (float)((r >> 16) & 0xFFu); }
+    static __device__ __forceinline__ bool has_w(T r) { return (r & 0x00FF0000u) != 0u; }
+    static __device__ __forceinline__ T make(float v, float wt) {
+        return (uint32_t)__half_as_ushort(__float2half_rn(v)) | ((uint32_t)(uint8_t)wt << 16);
+    }
+};
+__host__ __device__ constexpr int vox_rec_bytes(int tsdf_f16, int weight_u8) { return (tsdf_f16 && weight_u8) ? 4 : 8; }
+
+template <class TV, class TW>
+__device__ __forceinline__ typename VoxRec<TV, TW>::T ld_rec(const void *base, size_t i) {
+    return ((const VS_GLOBAL typename VoxRec<TV, TW>::T *)base)[i];
 }
-__device__ __forceinline__ void st_val(float *p, size_t i, float v) { stg(p, i, v); }
-__device__ __forceinline__ void st_val(__half *p, size_t i, float v) {
-    stg((uint16_t *)p, i, (uint16_t)__half_as_ushort(__float2half_rn(v)));
+// the same with a 32-bit byte offset from the base (records below 4 GB)
+template <class TV, class TW>
+__device__ __forceinline__ typename VoxRec<TV, TW>::T ld_rec32(const void *base, uint32_t i) {
+    typedef typename VoxRec<TV, TW>::T T;
+    return *(const VS_GLOBAL T *)((const VS_GLOBAL char *)base + i * (uint32_t)sizeof(T));
 }
-__device__ __forceinline__ float ld_w(const float *p, size_t i) { return ldg(p, i); }
-__device__ __forceinline__ float ld_w(const uint8_t *p, size_t i) { return (float)ldg(p, i); }
-template <class T>
-__device__ __forceinline__ T ldg_off(const T *p, uint32_t byte_off) {
-    return *(const VS_GLOBAL T *)((const VS_GLOBAL char *)p + byte_off);
+template <class TV, class TW>
+__device__ __forceinline__ void st_rec(void *base, size_t i, typename VoxRec<TV, TW>::T r) {
+    ((VS_GLOBAL typename VoxRec<TV, TW>::T *)base)[i] = r;
 }
-__device__ __forceinline__ float ld_val32(const float *p, uint32_t i) { return ldg_off(p, i << 2); }
-__device__ __forceinline__ float ld_val32(const __half *p, uint32_t i) {
-    return __half2float(__ushort_as_half(ldg_off((const uint16_t *)p, i << 1)));
+// value / weight of record i, and a record written from floats
+template <class TV, class TW>
+__device__ __forceinline__ float ld_val(const void *base, size_t i) {
+    return VoxRec<TV, TW>::val(ld_rec<TV, TW>(base, i));
 }
-__device__ __forceinline__ float ld_w32(const float *p, uint32_t i) { return ldg_off(p, i << 2); }
-__device__ __forceinline__ float ld_w32(const uint8_t *p, uint32_t i) { return (float)ldg_off(p, i); }
-__device__ __forceinline__ void st_w(float *p, size_t i, float v) { stg(p, i, v); }
-__device__ __forceinline__ void st_w(uint8_t *p, size_t i, float v) { stg(p, i, (uint8_t)v); }
+template <class TV, class TW>
+__device__ __forceinline__ float ld_w(const void *base, size_t i) {
+    return VoxRec<TV, TW>::w(ld_rec<TV, TW>(base, i));
+}
+template <class TV, class TW>
+__device__ __forceinline__ void st_vw(void *base, size_t i, float v, float w) {
+    st_rec<TV, TW>(base, i, VoxRec<TV, TW>::make(v, w));
+}
 
 }  // namespace vs

@@ -52,8 +52,8 @@ void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, int nsess
 // sd_slices: the most superbricks along z of any session (distance field)
 void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_type, int weight_type, int grid,
                       int sd_slices, hipStream_t s);
-// o32: every session's value and weight arrays are below 4 GB (the shading
-// then addresses them with 32-bit offsets)
+// o32: every session's voxel records are below 4 GB (the shading then
+// addresses them with 32-bit offsets)
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
                     bool o32, hipStream_t s);
 // a single session's model maps of `level` (npx pixels) into caller buffers

@@ -3,13 +3,15 @@
 //
 // Volume layout in HBM: brick-major. The cube is cut into 8^3 bricks; brick
 // (bx,by,bz) has index (bz*BH + by)*BW + bx and its 512 voxels are stored
-// contiguously, z-major then y then x inside the brick. A brick of FP32 values
-// is 2 KB, of FP16 values 1 KB, so one workgroup streams a whole brick with
-// fully coalesced accesses, and the 2x2x2 trilinear and 32-tap gradient
+// contiguously, z-major then y then x inside the brick, each as one record of
+// its value and weight (VoxRec: 4 bytes for FP16 + u8, 8 otherwise; a lookup
+// is one load). A brick is 2 or 4 KB, so one workgroup streams a whole brick
+// with fully coalesced accesses, and the 2x2x2 trilinear and 32-tap gradient
 // stencils of the raycast mostly stay inside one or two bricks.
 //
 // Per-session bitmaps: `bvis` (bricks touched by the current frame, rebuilt
-// each frame from the per-brick flags `vmask`), `bocc` (bricks holding any voxel of weight > 0; weights never
+// each frame from the per-brick flags `vmask`), `bocc` (bricks holding any
+// voxel of weight > 0; weights never
 // decrease, so the bit is set once and kept) and the band bits `bband` /
 // `sband` (bricks / superbricks that may hold a voxel of weight > 0 and a value
 // other than 1.0; set when an update writes such a value, and kept).
@@ -206,7 +208,7 @@ __device__ __forceinline__ bool gii(const uint8_t *plane, int W, int H, float xf
 // when the voxel was updated.
 template <class TV, class TW>
 __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const float *T, const float *Tcg, f3 g,
-                                                      TV *F, TW *Wt, uint32_t *col, int li, bool &band) {
+                                                      void *vb, uint32_t *col, int li, bool &band) {
     const LevelDesc &L = P.lev[0];
     const f3 p = transform(T, g);
     float u, v;
@@ -222,14 +224,15 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     const float norm = sqrtf(dx * dx + dy * dy + dz * dz);
     const float sdf = dm - lambda_inv * norm;
     if (sdf < -P.mu) return false;
-    const float w = ld_w(Wt, li);
+    typedef VoxRec<TV, TW> R;
+    const typename R::T rec = ld_rec<TV, TW>(vb, li);
+    const float w = R::w(rec);
     if (w >= P.max_w) return false;  // frozen voxel
     const float nw = w + 1.0f, inw = 1.0f / nw;
     const float tsd = fminf(1.0f, sdf / P.mu);
-    const float nv = (w * ld_val(F, li) + tsd) * inw;
+    const float nv = (w * R::val(rec) + tsd) * inw;
     band |= nv != 1.0f;
-    st_val(F, li, nv);
-    st_w(Wt, li, nw);
+    st_rec<TV, TW>(vb, li, R::make(nv, nw));
     uint32_t c = ldg(col, li);
     const size_t plane = (size_t)L.W * L.H;
 #pragma unroll
@@ -244,39 +247,30 @@ __device__ __forceinline__ bool integrate_voxel_color(const SessParams &P, const
     return true;
 }
 
-// Four consecutive voxels of a brick (x, x+1, x+2, x+3 of one row, x a
-// multiple of 4): one 16-, 8- or 4-byte access.
-__device__ __forceinline__ void ld4(const float *p, size_t i, float o[4]) {
-    const vs_f4 v = *(const VS_GLOBAL vs_f4 *)(p + i);
-    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+// Four consecutive voxel records of a brick (x, x+1, x+2, x+3 of one row, x a
+// multiple of 4): one 16-byte access (4-byte records) or two (8-byte).
+template <class TV, class TW>
+__device__ __forceinline__ void ld_rec4(const void *base, size_t i, typename VoxRec<TV, TW>::T r[4]) {
+    typedef typename VoxRec<TV, TW>::T T;
+    const VS_GLOBAL vs_u4 *p = (const VS_GLOBAL vs_u4 *)((const T *)base + i);
+    if constexpr (sizeof(T) == 4) {
+        const vs_u4 v = p[0];
+        r[0] = v.x; r[1] = v.y; r[2] = v.z; r[3] = v.w;
+    } else {
+        const vs_u4 a = p[0], b = p[1];
+        r[0] = T{a.x, a.y}; r[1] = T{a.z, a.w}; r[2] = T{b.x, b.y}; r[3] = T{b.z, b.w};
+    }
 }
-typedef unsigned int vs_u2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void ld4(const __half *p, size_t i, float o[4]) {
-    const vs_u2 v = *(const VS_GLOBAL vs_u2 *)((const uint16_t *)p + i);
-    o[0] = __half2float(__ushort_as_half((unsigned short)(v.x & 0xFFFFu)));
-    o[1] = __half2float(__ushort_as_half((unsigned short)(v.x >> 16)));
-    o[2] = __half2float(__ushort_as_half((unsigned short)(v.y & 0xFFFFu)));
-    o[3] = __half2float(__ushort_as_half((unsigned short)(v.y >> 16)));
-}
-__device__ __forceinline__ void ld4(const uint8_t *p, size_t i, float o[4]) {
-    const uint32_t v = *(const VS_GLOBAL uint32_t *)(p + i);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = (float)((v >> (8 * k)) & 255u);
-}
-__device__ __forceinline__ void st4(float *p, size_t i, const float v[4]) {
-    *(VS_GLOBAL vs_f4 *)(p + i) = vs_f4{v[0], v[1], v[2], v[3]};
-}
-__device__ __forceinline__ void st4(__half *p, size_t i, const float v[4]) {
-    uint32_t h[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __half_as_ushort(__float2half_rn(v[k]));
-    *(VS_GLOBAL vs_u2 *)((uint16_t *)p + i) = vs_u2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
-}
-__device__ __forceinline__ void st4(uint8_t *p, size_t i, const float v[4]) {
-    uint32_t w = 0u;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) w |= (uint32_t)(uint8_t)v[k] << (8 * k);
-    *(VS_GLOBAL uint32_t *)(p + i) = w;
+template <class TV, class TW>
+__device__ __forceinline__ void st_rec4(void *base, size_t i, const typename VoxRec<TV, TW>::T r[4]) {
+    typedef typename VoxRec<TV, TW>::T T;
+    VS_GLOBAL vs_u4 *p = (VS_GLOBAL vs_u4 *)((T *)base + i);
+    if constexpr (sizeof(T) == 4) {
+        p[0] = vs_u4{r[0], r[1], r[2], r[3]};
+    } else {
+        p[0] = vs_u4{r[0].x, r[0].y, r[1].x, r[1].y};
+        p[1] = vs_u4{r[2].x, r[2].y, r[3].x, r[3].y};
+    }
 }
 
 // The integrate walks the brick list in chunks of kIntegChunk consecutive
@@ -293,10 +287,10 @@ constexpr int kXcds = 8;
 // One workgroup step takes two bricks, 128 threads each; a thread updates
 // four voxels of one brick row (x0..x0+3, y, z), so a wave covers 256
 // consecutive voxels (z-slices of 64). Voxel arithmetic per voxel as
-// g_ctv_Compute_TSDF_of_Voxels (volume_integrator.cu:154-262). Values and
-// weights are read and written in whole 128-voxel groups (32 threads): a
-// group with any updated voxel loads all of its voxels and stores them all
-// back, unchanged ones as they were, so its memory lines are written whole
+// g_ctv_Compute_TSDF_of_Voxels (volume_integrator.cu:154-262). Voxel records
+// are read and written in whole 128-voxel groups (32 threads, 4 records
+// each): a group with any updated voxel loads all of its records and stores
+// them all back, unchanged ones as they were, so its memory lines are written whole
 // (a partly written line costs the memory a read-modify-write); a group
 // without updates touches neither. Per-brick updated counts and band flags
 // go to LDS; after each chunk one barrier, then thread k < 8 sets brick k's
@@ -337,8 +331,7 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
             const float *T = Tg.m;
             const int b = it.y;
             const int bx = b % P.bdim[0], by = (b / P.bdim[0]) % P.bdim[1], bz = b / (P.bdim[0] * P.bdim[1]);
-            TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
-            TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
+            void *vb = (typename VoxRec<TV, TW>::T *)P.vox + (size_t)b * kBrickVox;
             const LevelDesc &L = P.lev[0];
             // voxel-centre world coordinates from the session's tables
             const float gy = ldg(P.wtab[1], by * kBrick + yl), gz = ldg(P.wtab[2], bz * kBrick + zl);
@@ -357,7 +350,7 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
                 uint32_t *col = P.vcol + (size_t)b * kBrickVox;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    nupd += integrate_voxel_color(P, T, Tc.m, {gxs[q], gy, gz}, F, Wt, col, li + q, band);
+                    nupd += integrate_voxel_color<TV, TW>(P, T, Tc.m, {gxs[q], gy, gz}, vb, col, li + q, band);
             } else {
                 float tsd[4];
                 bool up[4];
@@ -377,20 +370,18 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
                 }
                 const unsigned long long any = __ballot(nupd > 0);
                 if ((any >> (lane & 32)) & 0xFFFFFFFFull) {
-                    float f[4], w[4];
-                    ld4(F, li, f);
-                    ld4(Wt, li, w);
+                    typedef VoxRec<TV, TW> R;
+                    typename R::T r[4];
+                    ld_rec4<TV, TW>(vb, li, r);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         if (!up[q]) continue;
-                        const float nw = w[q] + 1.0f;
-                        const float nv = __fdividef(fmaf(w[q], f[q], tsd[q]), nw);
+                        const float w = R::w(r[q]), nw = w + 1.0f;
+                        const float nv = __fdividef(fmaf(w, R::val(r[q]), tsd[q]), nw);
                         band |= nv != 1.0f;
-                        f[q] = nv;
-                        w[q] = fminf(nw, P.max_w);
+                        r[q] = R::make(nv, fminf(nw, P.max_w));
                     }
-                    st4(F, li, f);
-                    st4(Wt, li, w);
+                    st_rec4<TV, TW>(vb, li, r);
                 }
             }
             const unsigned c1 = (unsigned)__popcll(__ballot(nupd >= 1)), c2 = (unsigned)__popcll(__ballot(nupd >= 2));
@@ -536,8 +527,7 @@ void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipS
 template <class TV, class TW>
 struct Vol {
     const SessParams &P;
-    const TV *F;
-    const TW *Wt;
+    const void *vx;  // voxel records
     // superbrick distance field: the workgroup's LDS copy (4 bits per
     // superbrick). A session without one gets zeros (every superbrick counts
     // as a band one and only the brick bits decide); the index is masked to
@@ -569,21 +559,20 @@ struct Vol {
     // Value of an in-grid voxel. Weight and value are loaded together (no
     // dependency between the two loads); no weight reads as "no data" (1.0).
     __device__ __forceinline__ bool load(size_t idx, float &out) const {
-        const float w = ld_w(Wt, idx);
-        const float f = ld_val(F, idx);
-        const bool ok = w > 0.0f;
-        out = ok ? f : 1.0f;
+        typedef VoxRec<TV, TW> R;
+        const typename R::T r = ld_rec<TV, TW>(vx, idx);
+        const bool ok = R::has_w(r);
+        out = ok ? R::val(r) : 1.0f;
         return ok;
     }
-    // voxel (x, y, z) in range; O32: 32-bit offsets (arrays below 4 GB)
+    // voxel (x, y, z) in range; O32: 32-bit offsets (records below 4 GB)
     template <bool O32>
     __device__ __forceinline__ bool load_at(int x, int y, int z, float &out) const {
         if (!O32) return load(vox_index(P, x, y, z), out);
-        const uint32_t idx = vox_index32(P, x, y, z);
-        const float w = ld_w32(Wt, idx);
-        const float f = ld_val32(F, idx);
-        const bool ok = w > 0.0f;
-        out = ok ? f : 1.0f;
+        typedef VoxRec<TV, TW> R;
+        const typename R::T r = ld_rec32<TV, TW>(vx, vox_index32(P, x, y, z));
+        const bool ok = R::has_w(r);
+        out = ok ? R::val(r) : 1.0f;
         return ok;
     }
     // d_gtvu, exactly: nearest voxel by ROUNDF; true when it has weight.
@@ -639,9 +628,9 @@ struct Vol {
     // value otherwise; true when the tap succeeds.
     __device__ __forceinline__ bool ref_tap(int x, int y, int z, unsigned long long &bytes) const {
         if (!in_grid(x, y, z)) return false;
-        const float w = ld_w(Wt, vox_index(P, x, y, z));
+        const bool w = VoxRec<TV, TW>::has_w(ld_rec<TV, TW>(vx, vox_index(P, x, y, z)));
         bytes += sizeof(TW);
-        if (!(w > 0.0f)) return false;
+        if (!w) return false;
         bytes += sizeof(TV);
         return true;
     }
@@ -972,7 +961,7 @@ __device__ __forceinline__ uint32_t rgb_interpolated(const Vol<TV, TW> &V, const
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const size_t idx = vox_index(V.P, ox + (k & 1), oy + ((k >> 1) & 1), oz + (k >> 2));
-        ok &= ld_w(V.Wt, idx) > 0.0f;
+        ok &= VoxRec<TV, TW>::has_w(ld_rec<TV, TW>(V.vx, idx));
         c[k] = ldg(col, idx);
     }
     if (!ok) return 0u;
@@ -1065,7 +1054,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     C.step_scale = P.step_scale;
     C.total_max = P.total_max;
     C.max_iter = P.max_iter;
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, (const VS_LDS uint8_t *)s_dist};
+    const Vol<TV, TW> V{P, P.vox, (const VS_LDS uint8_t *)s_dist};
     RayStats rs;
     f3 p;
     const M34 Tc = ld_m34(st->T_cg_f);
@@ -1094,8 +1083,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
 // loads in flight); the same results for each, so the autotuner picks. (6
 // waves, 80 VGPRs, spills a few registers and was slower still.) O32: the
 // stencil's loads take 32-bit offsets from the array bases (every session
-// of the launch has value and weight arrays below 4 GB): 80 VGPRs instead of
-// 126 at WV 4, shading 0.93 -> 0.89 ms; the same results.
+// of the launch has its voxel records below 4 GB): 62 VGPRs instead of 101 at
+// WV 4, the same results.
 template <class TV, class TW, bool STATS, int RB, int WV, bool O32>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(WV == 4 ? 1 : WV, WV == 4 ? 8 : WV)))
 void k_raycast_shade(
@@ -1116,7 +1105,7 @@ void k_raycast_shade(
     // or -100 for a miss, which is final already
     const MapXYZ h = ld_mvtx(P.mvn[level], t);
     if (h.x == -100.0f) return;
-    const Vol<TV, TW> V{P, (const TV *)P.tsdf, (const TW *)P.weight, nullptr};
+    const Vol<TV, TW> V{P, P.vox, nullptr};
     const f3 p = {h.x, h.y, h.z};
     if (STATS) {
         unsigned long long b = 0;
@@ -1155,11 +1144,9 @@ void k_raycast_shade(
 template <class TV, class TW>
 __global__ void k_volume_init(const SessParams *__restrict__ SP, size_t nvox, uint32_t nwords) {
     const SessParams &P = SP[0];
-    TV *F = (TV *)P.tsdf;
-    TW *Wt = (TW *)P.weight;
+    const typename VoxRec<TV, TW>::T empty = VoxRec<TV, TW>::make(1.0f, 0.0f);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
-        st_val(F, i, 1.0f);
-        st_w(Wt, i, 0.0f);
+        st_rec<TV, TW>(P.vox, i, empty);
         if (P.vcol) stg(P.vcol, i, 0u);
     }
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += (size_t)gridDim.x * blockDim.x) {
@@ -1183,8 +1170,9 @@ __global__ void k_volume_to_linear(const SessParams *__restrict__ SP, size_t nvo
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
         const int z = (int)(i / WH), y = (int)((i / P.vdim[0]) % P.vdim[1]), x = (int)(i % P.vdim[0]);
         const size_t j = vox_index(P, x, y, z);
-        if (F) F[i] = ld_val((const TV *)P.tsdf, j);
-        if (Wt) Wt[i] = ld_w((const TW *)P.weight, j);
+        const typename VoxRec<TV, TW>::T r = ld_rec<TV, TW>(P.vox, j);
+        if (F) F[i] = VoxRec<TV, TW>::val(r);
+        if (Wt) Wt[i] = VoxRec<TV, TW>::w(r);
     }
 }
 // Whole bricks between the volume planes and a packed array (snapshots):
@@ -1199,19 +1187,42 @@ __global__ void k_brick_pack(const SessParams *__restrict__ SP, const uint32_t *
                              int sc, uint8_t *out, int unpack) {
     const SessParams &P = SP[0];
     const size_t bv = (size_t)kBrickVox * sv, bw = (size_t)kBrickVox * sw, bc = (size_t)kBrickVox * sc;
+    const bool r4 = vox_rec_bytes(sv == 2, sw == 1) == 4;
     for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
         const size_t b = idx[i];
         uint8_t *pv = out + i * bv, *pw = out + n * bv + i * bw, *pc = out + n * (bv + bw) + i * bc;
-        uint8_t *vv = (uint8_t *)P.tsdf + b * bv, *vw = (uint8_t *)P.weight + b * bw;
         uint8_t *vc = sc ? (uint8_t *)P.vcol + b * bc : nullptr;
-        if (unpack) {
-            copy16(vv, pv, bv);
-            copy16(vw, pw, bw);
-            if (sc) copy16(vc, pc, bc);
-        } else {
-            copy16(pv, vv, bv);
-            copy16(pw, vw, bw);
-            if (sc) copy16(pc, vc, bc);
+        // values and weights as separate planes of raw storage bits: the
+        // record's value bits (bits 0-15 or word 0) and weight bits (bits
+        // 16-23, or word 1)
+        for (int v = threadIdx.x; v < kBrickVox; v += blockDim.x) {
+            const size_t j = b * kBrickVox + v;
+            if (r4) {
+                uint32_t *rec = (uint32_t *)P.vox + j;
+                if (unpack)
+                    stg(rec, 0, (uint32_t)ldg((const uint16_t *)pv, v) | ((uint32_t)ldg(pw, v) << 16));
+                else {
+                    const uint32_t r = ldg(rec, 0);
+                    stg((uint16_t *)pv, v, (uint16_t)(r & 0xFFFFu));
+                    stg(pw, v, (uint8_t)(r >> 16));
+                }
+            } else {
+                uint32_t *rec = (uint32_t *)P.vox + 2 * j;
+                if (unpack) {
+                    stg(rec, 0, sv == 4 ? ldg((const uint32_t *)pv, v) : (uint32_t)ldg((const uint16_t *)pv, v));
+                    stg(rec, 1, sw == 4 ? ldg((const uint32_t *)pw, v) : (uint32_t)ldg(pw, v));
+                } else {
+                    const uint32_t r0 = ldg(rec, 0), r1 = ldg(rec, 1);
+                    if (sv == 4) stg((uint32_t *)pv, v, r0);
+                    else stg((uint16_t *)pv, v, (uint16_t)r0);
+                    if (sw == 4) stg((uint32_t *)pw, v, r1);
+                    else stg(pw, v, (uint8_t)r1);
+                }
+            }
+        }
+        if (sc) {
+            if (unpack) copy16(vc, pc, bc);
+            else copy16(pc, vc, bc);
         }
     }
 }
@@ -1246,13 +1257,10 @@ __global__ __launch_bounds__(kTileThreads) void k_carve(const SessParams *__rest
             any |= ldg(valid, ((size_t)z * P.vdim[1] + y) * P.vdim[0] + x) != 0;
         }
         if (__syncthreads_or(any)) continue;
-        TV *F = (TV *)P.tsdf + (size_t)b * kBrickVox;
-        TW *Wt = (TW *)P.weight + (size_t)b * kBrickVox;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int li = t + kTileThreads * h;
-            st_val(F, li, 1.0f);
-            st_w(Wt, li, 0.0f);
+            st_vw<TV, TW>(P.vox, (size_t)b * kBrickVox + li, 1.0f, 0.0f);
             if (P.vcol) stg(P.vcol, (size_t)b * kBrickVox + li, 0u);
         }
         if (t == 0) {
@@ -1288,8 +1296,7 @@ __global__ void k_volume_from_linear(const SessParams *__restrict__ SP, size_t n
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
         const int z = (int)(i / WH), y = (int)((i / P.vdim[0]) % P.vdim[1]), x = (int)(i % P.vdim[0]);
         const size_t j = vox_index(P, x, y, z);
-        st_val((TV *)P.tsdf, j, F[i]);
-        st_w((TW *)P.weight, j, Wt[i]);
+        st_vw<TV, TW>(P.vox, j, F[i], Wt[i]);
     }
 }
 // bocc and the band bits from the volume (after an upload or a restore): one
@@ -1298,15 +1305,14 @@ template <class TV, class TW>
 __global__ __launch_bounds__(kTileThreads) void k_occupancy(const SessParams *__restrict__ SP, uint32_t nbricks) {
     const SessParams &P = SP[0];
     for (uint32_t b = blockIdx.x; b < nbricks; b += gridDim.x) {
-        const TW *Wt = (const TW *)P.weight + (size_t)b * kBrickVox;
-        const TV *F = (const TV *)P.tsdf + (size_t)b * kBrickVox;
         bool any = false, band = false;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int i = threadIdx.x + kTileThreads * h;
-            const bool w = ld_w(Wt, i) > 0.0f;
+            const size_t i = (size_t)b * kBrickVox + threadIdx.x + kTileThreads * h;
+            const typename VoxRec<TV, TW>::T r = ld_rec<TV, TW>(P.vox, i);
+            const bool w = VoxRec<TV, TW>::has_w(r);
             any |= w;
-            band |= w && ld_val(F, i) != 1.0f;
+            band |= w && VoxRec<TV, TW>::val(r) != 1.0f;
         }
         const int cany = __syncthreads_or(any), cband = __syncthreads_or(band);
         if (threadIdx.x == 0) {
