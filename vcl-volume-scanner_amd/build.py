@@ -30,6 +30,22 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
+# Instruction-scheduling strategy of the AMDGPU backend per source (the
+# default aims for occupancy): the ICP's latency-bound iteration gains from
+# the ILP-maximising schedule (ICP stage 1.98 -> 1.87 ms per step), the
+# raycast shading from the iterative one (0.63 -> 0.60 ms); the march (same
+# file as the shading) is unchanged within noise, and the bilateral filter
+# loses with either (0.19 -> 0.21-0.27 ms), so vs_img.hip keeps the default.
+FILE_FLAGS = {
+    "vs_icp.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "vs_vol.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+}
+
+
+def hip_flags(src):
+    return HIP_FLAGS + FILE_FLAGS.get(src, [])
+
+
 LIB = os.path.join(HERE, "libvscan.so")
 SCENE_LIB = os.path.join(HERE, "libvscan_scene.so")
 
@@ -58,7 +74,7 @@ def build(verbose=False, variant=None):
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", f"{tag}.o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *hip_flags(src), *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         for r in ex.map(_run, cmds):
             if verbose and (r.stdout or r.stderr):

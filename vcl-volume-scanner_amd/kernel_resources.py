@@ -27,7 +27,7 @@ def code_object(src):
     os.makedirs(build.BUILD, exist_ok=True)
     base = os.path.join(build.BUILD, src.replace(".hip", ""))
     bundle, co = base + ".dev.o", base + f".{build.ARCH}.co"
-    build._run([build.HIPCC, *build.HIP_FLAGS, "--offload-device-only", "-c", os.path.join(build.CSRC, src),
+    build._run([build.HIPCC, *build.hip_flags(src), "--offload-device-only", "-c", os.path.join(build.CSRC, src),
                 "-o", bundle])
     build._run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={bundle}",
                 f"--targets=hipv4-amdgcn-amd-amdhsa--{build.ARCH}", f"--output={co}"])
