@@ -394,7 +394,7 @@ def main():
     log(f"bench: {args.sessions} sessions, {W}x{H}, {args.volume}^3 FP16+u8, warmup {Wm}, steps {K}")
     ctx = vscan.Context(0)
     sessions, streams, bufs, host = make_sessions(ctx, args.sessions, frames, W, H, args.volume,
-                                                  0 if args.no_host_frames else K)
+                                                  0 if args.no_host_frames else 2 * K)
     stream = hiprt.Stream()
 
     def step(k):
@@ -502,19 +502,34 @@ def main():
     if host:
         # the same steps with frames handed over in host memory (the PCIe
         # upload of each session's 614 KB frame inside the timed region), on
-        # the frames that follow the ones fused so far
-        stream.sync()
-        t0 = time.perf_counter()
-        for i in range(K):
-            ctx.fuse_batch(sessions, [hf[i] for hf in host], is_host=True, stream=stream.h)
-        stream.sync()
-        hw = time.perf_counter() - t0
+        # the frames that follow the ones fused so far: K steps from ordinary
+        # (pageable) numpy arrays, then K from page-locked buffers
+        def host_run(frames_of):
+            stream.sync()
+            t0 = time.perf_counter()
+            for i in range(K):
+                ctx.fuse_batch(sessions, frames_of(i), is_host=True, stream=stream.h)
+            stream.sync()
+            return time.perf_counter() - t0
+
+        hw = host_run(lambda i: [hf[i] for hf in host])
+        pinned = []
+        for hf in host:
+            pb = hiprt.PinnedBuffer(hf[K:].shape, hf.dtype)
+            pb.array[...] = hf[K:]
+            pinned.append(pb)
+        pw = host_run(lambda i: [pb.array[i] for pb in pinned])
+        for pb in pinned:
+            pb.free()
         out["host_frames"] = {"value": round(n * K / hw, 2), "unit": "fused frames/s",
                               "ms_per_step": round(hw * 1e3 / K, 3),
+                              "pinned_value": round(n * K / pw, 2), "pinned_ms_per_step": round(pw * 1e3 / K, 3),
                               "definition": "same batched step with each session's uint16 frame passed in host "
                                             "memory (vs_fuse_batch is_host=1: one 614 KB upload per session inside "
-                                            "the timed region)"}
-        log(f"host frames: {out['host_frames']['value']} fused frames/s")
+                                            "the timed region); value: ordinary numpy arrays, pinned_value: "
+                                            "page-locked buffers (hipHostMalloc)"}
+        log(f"host frames: {out['host_frames']['value']} fused frames/s, "
+            f"pinned {out['host_frames']['pinned_value']}")
         del host
     if args.serve_seconds > 0:
         for s in sessions:  # the serving run opens its own sessions

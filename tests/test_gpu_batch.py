@@ -167,3 +167,51 @@ def test_wide_offsets_equal_32bit_offsets(ctx):
     finally:
         for s in (alone, mixed, big):
             s.close()
+
+
+def test_host_frames_equal_device_frames(ctx):
+    """Frames handed to vs_fuse_batch in host memory (ordinary numpy arrays or
+    page-locked buffers) go through the batch's two device staging areas in
+    turn; the results equal those of the same frames passed in device memory,
+    over consecutive steps (both areas, reused)."""
+    import hiprt
+    print("case: host frames (pageable, pinned) vs device frames", flush=True)
+    specs = [(4, 11, 320, 240, 96), (3, 12, 640, 480, 64), (3, 13, 320, 240, 64)]
+    made = _sessions(ctx, vscan.VS_TSDF_F16, vscan.VS_WEIGHT_U8, specs)
+    runs = {k: [vscan.Session(ctx, d, T_gc0=st.pose(0)) for st, d in made] for k in ("page", "pin", "dev")}
+    bufs = []
+    try:
+        for fr in range(5):
+            frames = [np.ascontiguousarray(st.frame(fr), np.uint16) for st, _ in made]
+            ctx.fuse_batch(runs["page"], frames)
+            pins = []
+            for f in frames:
+                pb = hiprt.PinnedBuffer(f.shape, np.uint16)
+                pb.array[...] = f
+                pins.append(pb)
+            ctx.fuse_batch(runs["pin"], [pb.array for pb in pins])
+            devs = []
+            for f in frames:
+                db = hiprt.DeviceBuffer(f.nbytes)
+                db.upload(f)
+                devs.append(db)
+            ctx.fuse_batch(runs["dev"], [db.ptr.value for db in devs], is_host=False)
+            for s in runs["dev"]:
+                s.pose()  # waits for the step
+            bufs += pins + devs
+            for a, b, c in zip(runs["page"], runs["pin"], runs["dev"]):
+                assert a.status() == b.status() == c.status()
+                assert np.array_equal(a.pose(), c.pose()) and np.array_equal(b.pose(), c.pose()), f"frame {fr}"
+        for a, b, c in zip(runs["page"], runs["pin"], runs["dev"]):
+            Fc, Wc, _ = c.volume()
+            for x in (a, b):
+                Fx, Wx, _ = x.volume()
+                assert np.array_equal(Fx, Fc) and np.array_equal(Wx, Wc)
+                for l in range(3):
+                    assert all(np.array_equal(u, v) for u, v in zip(x.maps(l), c.maps(l)))
+    finally:
+        for ss in runs.values():
+            for s in ss:
+                s.close()
+        for b in bufs:
+            b.free()

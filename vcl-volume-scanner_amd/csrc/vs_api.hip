@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "vs_device.hpp"
@@ -256,6 +257,24 @@ struct TuneState {
     ~TuneState() { timer.destroy(); }
 };
 
+// Frames that a batched step receives in host memory are copied by the CPU
+// into one page-locked area and uploaded by a single DMA transfer on a copy
+// stream of the library's own, then gathered into the sessions' raw-frame
+// buffers on the step's stream. Two areas (host and device) are used in
+// turn: the upload that one call enqueues waits only for the gather that
+// read the same device area two calls before, so it runs while the GPU
+// still fuses the previous step. (One transfer per frame costs the step
+// about 1 ms more at 64 frames of 614 KB, even from page-locked memory.)
+struct HostStage {
+    hipStream_t cs = nullptr;
+    char *pinned[2] = {nullptr, nullptr};  // host areas
+    char *area[2] = {nullptr, nullptr};    // device areas
+    size_t bytes = 0;  // capacity of each area
+    hipEvent_t copied[2] = {nullptr, nullptr}, read[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};  // copied[g] and read[g] recorded since the areas were (re)allocated
+    int next = 0;
+};
+
 struct Batch {
     int n = 0, tt = 0, wt = 0, levels = 0;
     int max_iters[kMaxLevels] = {0, 0, 0};
@@ -284,6 +303,7 @@ struct Batch {
     // (the last workgroups of a launch, a long ray) overlap the other parts'
     // work. Per-session results do not depend on the grouping.
     std::vector<std::unique_ptr<Batch>> sub;
+    HostStage hs;  // host-memory frames of this batch's steps
 };
 
 struct vs_ctx {
@@ -495,8 +515,22 @@ void free_batch_graph(Batch &b) {
     b.graph = nullptr;
 }
 
+void release_stage(HostStage &S) {
+    if (S.cs) hipStreamSynchronize(S.cs);
+    for (int g = 0; g < 2; ++g) {
+        if (S.read[g]) hipEventSynchronize(S.read[g]);
+        if (S.area[g]) hipFree(S.area[g]);
+        if (S.pinned[g]) hipHostFree(S.pinned[g]);
+        if (S.copied[g]) hipEventDestroy(S.copied[g]);
+        if (S.read[g]) hipEventDestroy(S.read[g]);
+    }
+    if (S.cs) hipStreamDestroy(S.cs);
+    S = HostStage();
+}
+
 // Graph, allocation and halves of a context batch.
 void release_batch(Batch &b) {
+    release_stage(b.hs);
     free_batch_graph(b);
     if (b.owned) hipFree(b.owned);
     b.owned = nullptr;
@@ -904,6 +938,90 @@ vs_status copy_depth(vs_session *s, const void *depth, int is_host, hipStream_t 
     const size_t bytes = frame_bytes(s->d);
     HIP_TRY(hipMemcpyAsync((void *)s->hp.raw, depth, bytes, is_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                            st));
+    return VS_OK;
+}
+
+// Host-memory frames of a batched step, frame k for batch slot slot[k]
+// (slot = k when null) of session sess[k]; a null frame keeps that session's
+// raw frame as it is. The CPU copies the frames into the stage's next host
+// area (several threads for a large batch), one transfer moves the area to
+// the device on the copy stream, and one gather per 64 frames on stream h
+// moves them into the raw-frame buffers once the transfer has run. The
+// caller's buffers are free again when this returns.
+vs_status stage_host_frames(HostStage &S, const SessParams *dP, vs_session *const *sess, const int *slot,
+                            const uint16_t *const *depths, int n, hipStream_t h) {
+    struct Job {
+        const void *src;
+        size_t off, bytes;
+        int slot;
+    };
+    std::vector<Job> jobs;
+    size_t need = 0;
+    for (int k = 0; k < n; ++k) {
+        if (!depths[k]) continue;
+        const size_t bytes = frame_bytes(sess[k]->d);
+        jobs.push_back({depths[k], need, bytes, slot ? slot[k] : k});
+        need += (bytes + 255) & ~(size_t)255;
+    }
+    if (need == 0) return VS_OK;
+    if (!S.cs) {
+        HIP_TRY(hipStreamCreateWithFlags(&S.cs, hipStreamNonBlocking));
+        for (int g = 0; g < 2; ++g) {
+            HIP_TRY(hipEventCreateWithFlags(&S.copied[g], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&S.read[g], hipEventDisableTiming));
+        }
+    }
+    if (need > S.bytes) {
+        HIP_TRY(hipStreamSynchronize(S.cs));
+        for (int g = 0; g < 2; ++g) {
+            if (S.used[g]) HIP_TRY(hipEventSynchronize(S.read[g]));
+            S.used[g] = false;
+            if (S.area[g]) HIP_TRY(hipFree(S.area[g]));
+            if (S.pinned[g]) HIP_TRY(hipHostFree(S.pinned[g]));
+            S.area[g] = S.pinned[g] = nullptr;
+        }
+        S.bytes = 0;
+        for (int g = 0; g < 2; ++g) {
+            HIP_TRY(hipMalloc((void **)&S.area[g], need));
+            HIP_TRY(hipHostMalloc((void **)&S.pinned[g], need, hipHostMallocDefault));
+        }
+        S.bytes = need;
+    }
+    const int g = S.next;
+    S.next ^= 1;
+    // the transfer that last read this host area has run
+    if (S.used[g]) HIP_TRY(hipEventSynchronize(S.copied[g]));
+    char *dst = S.pinned[g];
+    const int nt = (int)std::min<size_t>({(size_t)8, std::max(1u, std::thread::hardware_concurrency()),
+                                          std::max<size_t>(1, need >> 22)});  // a thread per 4 MB, at most 8
+    auto copy_range = [&](int t) {
+        for (size_t j = t; j < jobs.size(); j += nt) memcpy(dst + jobs[j].off, jobs[j].src, jobs[j].bytes);
+    };
+    if (nt > 1) {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(copy_range, t);
+        copy_range(0);
+        for (auto &th : pool) th.join();
+    } else {
+        copy_range(0);
+    }
+    if (S.used[g]) HIP_TRY(hipStreamWaitEvent(S.cs, S.read[g], 0));  // the gather that read this device area
+    HIP_TRY(hipMemcpyAsync(S.area[g], dst, need, hipMemcpyHostToDevice, S.cs));
+    HIP_TRY(hipEventRecord(S.copied[g], S.cs));
+    HIP_TRY(hipStreamWaitEvent(h, S.copied[g], 0));
+    DepthChunk ch{};
+    for (const Job &j : jobs) {
+        ch.src[ch.n] = (const uint16_t *)(S.area[g] + j.off);
+        ch.slot[ch.n] = j.slot;
+        if (++ch.n == kDepthChunk) {
+            launch_gather_depth(dP, ch, h);
+            ch.n = 0;
+        }
+    }
+    launch_gather_depth(dP, ch, h);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(S.read[g], h));
+    S.used[g] = true;
     return VS_OK;
 }
 
@@ -1479,10 +1597,8 @@ vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t
         B = it->second.get();
     }
     if (depths && is_host) {
-        for (int i = 0; i < n; ++i) {
-            vs_status r = copy_depth(key[i], depths[i], 1, h);
-            if (r != VS_OK) return r;
-        }
+        vs_status r = stage_host_frames(B->hs, B->dP, key.data(), nullptr, depths, n, h);
+        if (r != VS_OK) return r;
     } else if (depths) {
         // device-resident frames: one gather launch per 64 sessions that have one
         DepthChunk ch{};
@@ -2080,6 +2196,7 @@ vs_status vs_server_destroy(vs_server *v) {
         if (v->lists[g]) hipFreeAsync(v->lists[g], server_order(v));
     (void)server_quiesce(v);
     server_reap(v, true);
+    release_stage(v->B.hs);
     free_batch_graph(v->B);
     for (int g = 0; g < 2; ++g) {
         if (v->lstage[g]) hipHostFree(v->lstage[g]);
@@ -2150,10 +2267,10 @@ vs_status vs_server_step(vs_server *v, const int *slots, const uint16_t *const *
         HIP_TRY(hipStreamWaitEvent(h, v->order, 0));
     }
     if (depths && is_host) {
-        for (int k = 0; k < n; ++k) {
-            vs_status r = copy_depth(v->slot[slots[k]], depths[k], 1, h);
-            if (r != VS_OK) return r;
-        }
+        std::vector<vs_session *> ss(n);
+        for (int k = 0; k < n; ++k) ss[k] = v->slot[slots[k]];
+        vs_status r = stage_host_frames(v->B.hs, v->B.dP, ss.data(), slots, depths, n, h);
+        if (r != VS_OK) return r;
     } else if (depths) {
         DepthChunk ch{};
         for (int k = 0; k < n; ++k) {

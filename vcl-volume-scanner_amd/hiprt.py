@@ -20,10 +20,13 @@ def hip():
         L = C.CDLL(path, mode=C.RTLD_GLOBAL)
         for f in ["hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize", "hipStreamCreate",
                   "hipStreamDestroy", "hipStreamSynchronize", "hipEventCreate", "hipEventDestroy", "hipEventRecord",
-                  "hipEventSynchronize", "hipEventElapsedTime", "hipGetDeviceCount", "hipMemGetInfo", "hipEventQuery"]:
+                  "hipEventSynchronize", "hipEventElapsedTime", "hipGetDeviceCount", "hipMemGetInfo", "hipEventQuery",
+                  "hipHostMalloc", "hipHostFree"]:
             getattr(L, f).restype = C.c_int
         L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.hipFree.argtypes = [C.c_void_p]
+        L.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        L.hipHostFree.argtypes = [C.c_void_p]
         L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
@@ -64,6 +67,26 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+class PinnedBuffer:
+    """Page-locked host memory (hipHostMalloc) with a numpy view: frames
+    handed to the library from here upload by DMA alone, with no staging
+    copy on the CPU."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        self.ptr = C.c_void_p()
+        _ok(hip().hipHostMalloc(C.byref(self.ptr), max(1, self.nbytes), 0), "hipHostMalloc")
+        raw = (C.c_uint8 * self.nbytes).from_address(self.ptr.value)
+        self.array = np.frombuffer(raw, dtype=dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            hip().hipHostFree(self.ptr)
             self.ptr = C.c_void_p()
 
 
