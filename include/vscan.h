@@ -211,7 +211,9 @@ vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs
 /* Many sessions, one frame each, in one batched launch per stage. Sessions
  * must share tsdf/weight types. depths[i] are device pointers unless is_host;
  * a NULL depths[i] (or depths NULL) fuses the frame already in that session's
- * raw buffer. */
+ * raw buffer. Host frames are copied before the call returns (the caller may
+ * reuse its buffers at once) and uploaded by one transfer on a copy stream of
+ * the library's own, which overlaps the previous step's GPU work. */
 vs_status vs_fuse_batch(vs_ctx *ctx, vs_session *const *sessions, const uint16_t *const *depths,
                         int n, int is_host, vs_stream stream);
 
@@ -401,8 +403,9 @@ vs_status vs_server_destroy(vs_server *srv);
 vs_status vs_server_open(vs_server *srv, const vs_session_desc *desc, const double T_gc0[16], int *slot);
 vs_status vs_server_close(vs_server *srv, int slot);
 vs_session *vs_server_session(vs_server *srv, int slot);
-/* depths[i] is slot slots[i]'s frame (device pointer unless is_host); each
- * slot at most once per step. Asynchronous on `stream` (NULL: the server's). */
+/* depths[i] is slot slots[i]'s frame (device pointer unless is_host; host
+ * frames as for vs_fuse_batch); each slot at most once per step. Asynchronous
+ * on `stream` (NULL: the server's). */
 vs_status vs_server_step(vs_server *srv, const int *slots, const uint16_t *const *depths, int n, int is_host,
                          vs_stream stream);
 /* Open slots and the session bytes they hold. */
