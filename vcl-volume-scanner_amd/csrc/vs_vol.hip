@@ -524,6 +524,18 @@ void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipS
 // the others hold exactly 1.0): the march reads such bricks as 1.0 without
 // touching the volume, and skips the steps that stay inside them.
 // ---------------------------------------------------------------------------
+// Range tests of the march's lookups, the three axes evaluated together
+// (bitwise, no short-circuit). finite3: every coordinate inside the range
+// where the reference's int conversions are defined (|v| < 2147483520);
+// below3: 0 <= a < na on every axis, unsigned compares.
+__device__ __forceinline__ bool finite3(f3 v) {
+    constexpr float kLim = 2147483520.0f;
+    return (int)(fabsf(v.x) < kLim) & (int)(fabsf(v.y) < kLim) & (int)(fabsf(v.z) < kLim);
+}
+__device__ __forceinline__ bool below3(int x, int y, int z, int nx, int ny, int nz) {
+    return (int)((unsigned)x < (unsigned)nx) & (int)((unsigned)y < (unsigned)ny) & (int)((unsigned)z < (unsigned)nz);
+}
+
 template <class TV, class TW>
 struct Vol {
     const SessParams &P;
@@ -594,11 +606,16 @@ struct Vol {
     // size = -1 when the position is outside the grid (moving towards it),
     // size = 0 otherwise.
     __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out, int3 &lo, int &size) const {
-        int x, y, z;
         out = 1.0f;
         size = 0;
-        if (!roundf_ref(v.x, x) || !roundf_ref(v.y, y) || !roundf_ref(v.z, z)) return 0;
-        if (!in_grid(x, y, z)) {
+        // the tests of the three axes are combined with & rather than &&:
+        // one exec-mask branch instead of a nest of six, each with its own
+        // mask bookkeeping (the march loop's instruction issue is its bound)
+        if (!finite3(v)) return 0;
+        const int x = v.x < 0.0f ? (int)(v.x - 0.5f) : (int)(v.x + 0.5f);  // ROUNDF
+        const int y = v.y < 0.0f ? (int)(v.y - 0.5f) : (int)(v.y + 0.5f);
+        const int z = v.z < 0.0f ? (int)(v.z - 0.5f) : (int)(v.z + 0.5f);
+        if (!below3(x, y, z, P.vdim[0], P.vdim[1], P.vdim[2])) {
             if ((x < 0 && !(rd.x > 0.0f)) || (x >= P.vdim[0] && !(rd.x < 0.0f)) || (y < 0 && !(rd.y > 0.0f)) ||
                 (y >= P.vdim[1] && !(rd.y < 0.0f)) || (z < 0 && !(rd.z > 0.0f)) ||
                 (z >= P.vdim[2] && !(rd.z < 0.0f)))
@@ -673,9 +690,10 @@ struct Vol {
     // no occupancy lookup is needed here.
     __device__ __forceinline__ bool trilinear(f3 v, float &out) const {
         out = 1.0f;
-        if (!(fabsf(v.x) < 2147483520.0f && fabsf(v.y) < 2147483520.0f && fabsf(v.z) < 2147483520.0f)) return false;
+        if (!finite3(v)) return false;
         const int ox = (int)v.x, oy = (int)v.y, oz = (int)v.z;
-        if (!in_grid(ox, oy, oz) || !in_grid(ox + 1, oy + 1, oz + 1)) return false;
+        // 0 <= o and o + 1 < vdim on every axis
+        if (!below3(ox, oy, oz, P.vdim[0] - 1, P.vdim[1] - 1, P.vdim[2] - 1)) return false;
         const float rx = v.x - (float)ox, ry = v.y - (float)oy, rz = v.z - (float)oz;
         float f[8];
         bool ok = true;
@@ -799,7 +817,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         if (!STATS && !coarse) {
             const unsigned bx = __float_as_uint(vox.x), by = __float_as_uint(vox.y), bz = __float_as_uint(vox.z),
                            bt = __float_as_uint(tsdf);
-            if (bx == sv_x && by == sv_y && bz == sv_z && bt == sv_t) return false;
+            if (((bx ^ sv_x) | (by ^ sv_y) | (bz ^ sv_z) | (bt ^ sv_t)) == 0u) return false;
             if ((brent & 0xFFFFu) == (1u << (brent >> 16))) {
                 sv_x = bx;
                 sv_y = by;
