@@ -616,9 +616,9 @@ struct Vol {
         const int y = v.y < 0.0f ? (int)(v.y - 0.5f) : (int)(v.y + 0.5f);
         const int z = v.z < 0.0f ? (int)(v.z - 0.5f) : (int)(v.z + 0.5f);
         if (!below3(x, y, z, P.vdim[0], P.vdim[1], P.vdim[2])) {
-            if ((x < 0 && !(rd.x > 0.0f)) || (x >= P.vdim[0] && !(rd.x < 0.0f)) || (y < 0 && !(rd.y > 0.0f)) ||
-                (y >= P.vdim[1] && !(rd.y < 0.0f)) || (z < 0 && !(rd.z > 0.0f)) ||
-                (z >= P.vdim[2] && !(rd.z < 0.0f)))
+            if ((int)(x < 0 && !(rd.x > 0.0f)) | (int)(x >= P.vdim[0] && !(rd.x < 0.0f)) |
+                (int)(y < 0 && !(rd.y > 0.0f)) | (int)(y >= P.vdim[1] && !(rd.y < 0.0f)) |
+                (int)(z < 0 && !(rd.z > 0.0f)) | (int)(z >= P.vdim[2] && !(rd.z < 0.0f)))
                 return -1;
             size = -1;
             return 0;
@@ -818,21 +818,22 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
             const unsigned bx = __float_as_uint(vox.x), by = __float_as_uint(vox.y), bz = __float_as_uint(vox.z),
                            bt = __float_as_uint(tsdf);
             if (((bx ^ sv_x) | (by ^ sv_y) | (bz ^ sv_z) | (bt ^ sv_t)) == 0u) return false;
-            if ((brent & 0xFFFFu) == (1u << (brent >> 16))) {
-                sv_x = bx;
-                sv_y = by;
-                sv_z = bz;
-                sv_t = bt;
-                brent = ((brent >> 16) + 1) << 16;
-            }
-            ++brent;
+            const bool save = (brent & 0xFFFFu) == (1u << (brent >> 16));
+            sv_x = save ? bx : sv_x;
+            sv_y = save ? by : sv_y;
+            sv_z = save ? bz : sv_z;
+            sv_t = save ? bt : sv_t;
+            brent = (save ? ((brent >> 16) + 1) << 16 : brent) + 1u;
         }
-        if (++iter > C.max_iter) return false;
-        const float step = coarse ? C.step_scale
-                                  : ((tsdf > 0.0f) ? fmaxf(C.step_scale * tsdf, 1.0f) : fminf(C.step_scale * tsdf, -1.0f));
+        // fine step: max(s*tsdf, 1) for tsdf > 0, else min(s*tsdf, -1), as
+        // +-max(|s*tsdf|, 1) (s > 0); both exits give a miss, so they share
+        // one test
+        ++iter;
+        const float sm = fmaxf(fabsf(C.step_scale * tsdf), 1.0f);
+        const float step = coarse ? C.step_scale : (tsdf > 0.0f ? sm : -sm);
         vox = add(vox, scale(step, rd));
         total += step;
-        if (total > C.total_max) return false;
+        if ((int)(iter > C.max_iter) | (int)(total > C.total_max)) return false;
         if (STATS) rs.nearest(V, vox);
         // A ray that has left the grid for good can only end as a miss
         // (every later tap reads 1.0 and the step stays positive); the
@@ -871,7 +872,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
             }
             continue;
         }
-        if (tsdf > -1.0f && tsdf < 1.0f) {
+        if (fabsf(tsdf) < 1.0f) {
             if (STATS) rs.trilinear(V, vox);
             V.trilinear(vox, tsdf);
             if (tsdf < 0.0f) {
