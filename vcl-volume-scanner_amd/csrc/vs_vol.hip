@@ -555,9 +555,14 @@ struct Vol {
     __device__ __forceinline__ int cb() const { return (int)(cbf & kIdx); }
     __device__ __forceinline__ int dist() const { return (int)((cbf >> 25) & 15u); }
     __device__ __forceinline__ bool band(int x, int y, int z) const {
-        const unsigned b = (unsigned)brick_of(P, x, y, z);
+        // brick_of and super_of_brick of an in-grid voxel with 24-bit
+        // products (< 2^24 bricks)
+        const unsigned b = __umul24(__umul24((unsigned)z >> 3, (unsigned)P.bdim[1]) + ((unsigned)y >> 3),
+                                    (unsigned)P.bdim[0]) + ((unsigned)x >> 3);
         if (b != (cbf & kIdx)) {
-            const unsigned sb = (unsigned)super_of_brick(P, x >> 3, y >> 3, z >> 3);
+            constexpr int ss = 3 + kSuperShift;
+            const unsigned sb = __umul24(__umul24((unsigned)z >> ss, (unsigned)P.sdim[1]) + ((unsigned)y >> ss),
+                                         (unsigned)P.sdim[0]) + ((unsigned)x >> ss);
             const unsigned d = (sdist[(sb >> 1) & (kDistCells / 2 - 1)] >> ((sb & 1u) * 4u)) & 15u;
             unsigned f = b | (d << 25);
             if (d == 0u && ((ldg(P.bband, b >> 5) >> (b & 31)) & 1u)) f |= kBrickBand;
@@ -624,16 +629,13 @@ struct Vol {
             return 0;
         }
         if (!band(x, y, z)) {
+            // d = 0: the brick (a brick without band bit in a band
+            // superbrick); d > 0: the (2d-1)^3 superbricks centred here,
+            // which hold no band brick
             const int d = dist();
-            if (d == 0) {  // a brick without band bit in a band superbrick
-                lo = make_int3(x & ~7, y & ~7, z & ~7);
-                size = kBrick;
-            } else {  // the (2d-1)^3 superbricks centred here hold no band brick
-                const int r = d - 1;
-                constexpr int sh = 3 + kSuperShift;  // voxel -> superbrick
-                lo = make_int3(((x >> sh) - r) * kSuper, ((y >> sh) - r) * kSuper, ((z >> sh) - r) * kSuper);
-                size = (2 * r + 1) * kSuper;
-            }
+            const int sh = d == 0 ? 3 : 3 + kSuperShift, r = max(d - 1, 0), unit = 1 << sh;
+            lo = make_int3(((x >> sh) - r) * unit, ((y >> sh) - r) * unit, ((z >> sh) - r) * unit);
+            size = (2 * r + 1) * unit;
             return 0;
         }
         return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
@@ -695,10 +697,29 @@ struct Vol {
         // 0 <= o and o + 1 < vdim on every axis
         if (!below3(ox, oy, oz, P.vdim[0] - 1, P.vdim[1] - 1, P.vdim[2] - 1)) return false;
         const float rx = v.x - (float)ox, ry = v.y - (float)oy, rz = v.z - (float)oz;
+        // vox_index of the 8 corners from per-axis parts: brick index
+        // bz*bdim1*bdim0 + by*bdim0 + bx (24-bit products: < 2^24 bricks)
+        // and in-brick offset lz*64 + ly*8 + lx of the two coordinates on
+        // each axis
+        const unsigned bd0 = (unsigned)P.bdim[0], bd01 = (unsigned)(P.bdim[0] * P.bdim[1]);
+        unsigned Bx[2], By[2], Bz[2], Lx[2], Ly[2], Lz[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            Bx[a] = (unsigned)(ox + a) >> 3;
+            By[a] = __umul24((unsigned)(oy + a) >> 3, bd0);
+            Bz[a] = __umul24((unsigned)(oz + a) >> 3, bd01);
+            Lx[a] = (unsigned)(ox + a) & 7u;
+            Ly[a] = ((unsigned)(oy + a) & 7u) << 3;
+            Lz[a] = ((unsigned)(oz + a) & 7u) << 6;
+        }
         float f[8];
         bool ok = true;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) ok &= load(vox_index(P, ox + (k & 1), oy + ((k >> 1) & 1), oz + (k >> 2)), f[k]);
+        for (int k = 0; k < 8; ++k) {
+            const unsigned b = Bz[k >> 2] + By[(k >> 1) & 1] + Bx[k & 1];
+            const unsigned l = Lz[k >> 2] | Ly[(k >> 1) & 1] | Lx[k & 1];
+            ok &= load((size_t)b * kBrickVox + l, f[k]);
+        }
         if (!ok) return false;
         const float fi = (1.0f - ry) * ((1.0f - rx) * f[0] + rx * f[1]) + ry * ((1.0f - rx) * f[2] + rx * f[3]);
         const float bi = (1.0f - ry) * ((1.0f - rx) * f[4] + rx * f[5]) + ry * ((1.0f - rx) * f[6] + rx * f[7]);
@@ -847,9 +868,8 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
             const float S = coarse ? C.step_scale : fmaxf(C.step_scale * tsdf, 1.0f);
             const f3 d = scale(S, rd);
             const int k = size > 0 ? steps_in_box(vox, d, lo, size, V.P.vdim) : steps_outside(vox, d, V.P.vdim);
-            if (k > 0) {
+            {  // k = 0 changes nothing
                 iter += k;
-                if (iter > C.max_iter) return false;
                 int j = k;
                 for (; j >= 4; j -= 4) {
 #pragma unroll
@@ -868,7 +888,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
                     vox = add(vox, d);
                     total += S;
                 }
-                if (total > C.total_max) return false;
+                if ((int)(iter > C.max_iter) | (int)(total > C.total_max)) return false;
             }
             continue;
         }
