@@ -359,12 +359,17 @@ __global__ __launch_bounds__(kTileThreads) void k_integrate(const SessParams *__
                     const f3 p = transform(T, {gxs[q], gy, gz});
                     float u, v;
                     project(p, L.K, u, v);
-                    int iu = 0, iv = 0;
-                    bool ok = roundf_ref(u, iu) && roundf_ref(v, iv);
-                    ok = ok && iu >= 0 && iu <= L.W - 1 && iv >= 0 && iv <= L.H - 1;
-                    const float dm = ok ? raw_depth(P, (size_t)iv * L.W + iu) : 0.0f;
+                    // the pixel tests combined with & (no nest of branches); the
+                    // depth is read at pixel 0 for a voxel outside the image
+                    const bool fin = (int)(fabsf(u) < 2147483520.0f) & (int)(fabsf(v) < 2147483520.0f);
+                    const float uc = fin ? u : 0.0f, vc = fin ? v : 0.0f;
+                    const int iu = uc < 0.0f ? (int)(uc - 0.5f) : (int)(uc + 0.5f);  // ROUNDF
+                    const int iv = vc < 0.0f ? (int)(vc - 0.5f) : (int)(vc + 0.5f);
+                    const bool ok = (int)fin & (int)((unsigned)iu < (unsigned)L.W) & (int)((unsigned)iv < (unsigned)L.H);
+                    const float d0 = raw_depth(P, ok ? (size_t)iv * L.W + iu : 0);
+                    const float dm = ok ? d0 : 0.0f;
                     const float td = dm - p.z;
-                    up[q] = ok && dm > 0.0f && !(td < -P.mu);
+                    up[q] = (int)ok & (int)(dm > 0.0f) & (int)!(td < -P.mu);
                     tsd[q] = fminf(1.0f, __fdividef(td, P.mu));
                     nupd += up[q];
                 }
