@@ -144,11 +144,11 @@ def test_failure_keeps_pose_and_skips_fusion(ctx):
 
 
 def test_wide_offsets_equal_32bit_offsets(ctx):
-    """The shading addresses the volume with 32-bit offsets when every
-    session of a launch has value and weight arrays below 4 GB, and with
-    64-bit addresses otherwise. A session batched with a 1024^3 FP32 session
-    (4 GB of values) takes the 64-bit path and must give the same bits as
-    the same session alone (32-bit path)."""
+    """The march and the shading address the volume with 32-bit offsets
+    when every session of a launch has its voxel records below 4 GB, and
+    with 64-bit addresses otherwise. A session batched with a 1024^3 FP32
+    session (8 GB of records) takes the 64-bit path and must give the same
+    bits as the same session alone (32-bit path)."""
     print("case: 128^3 session alone vs batched with a 1024^3 FP32 session, 4 frames", flush=True)
     (st, d), (stb, db) = _sessions(ctx, vscan.VS_TSDF_F32, vscan.VS_WEIGHT_F32,
                                    [(4, 1, 320, 240, 128), (3, 7, 320, 240, 1024)])

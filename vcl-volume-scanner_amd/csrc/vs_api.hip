@@ -732,7 +732,7 @@ void enqueue_step(const Batch &B, hipStream_t s, Timer *timed) {
     launch_integrate(B.dP, B.n, B.bl, B.tt, B.wt, integ_grid(B), B.sd_slices, s);
     mark(5);
     const LevelWork lw = level_work(B.dWork, B.nWork, B.levels);
-    launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, s);
+    launch_raycast_march(B.dP, lw, B.tt, B.wt, nullptr, B.shape.march, B.o32, s);
     mark(6);
     launch_raycast_shade(B.dP, lw, B.tt, B.wt, nullptr, kShadeOpts[B.shape.shade].block,
                          kShadeOpts[B.shape.shade].waves, B.o32, s);

@@ -52,8 +52,8 @@ void launch_vis_mark(const SessParams *P, const WorkItem *work, int n, int nsess
 // sd_slices: the most superbricks along z of any session (distance field)
 void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tsdf_type, int weight_type, int grid,
                       int sd_slices, hipStream_t s);
-// o32: every session's voxel records are below 4 GB (the shading then
-// addresses them with 32-bit offsets)
+// o32: every session's voxel records are below 4 GB (the march and the
+// shading then address them with 32-bit offsets)
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tsdf_type, int weight_type,
                     bool o32, hipStream_t s);
 // a single session's model maps of `level` (npx pixels) into caller buffers
@@ -79,8 +79,9 @@ LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLev
 void launch_sdist(const SessParams *P, int nsess, int sd_slices, int force, hipStream_t s);
 // Ray block of one wave in the raycast kernels (same results for each).
 enum RayBlock { kRay8x8 = 0, kRay16x4 = 1, kRay32x2 = 2 };
+// o32 as for launch_raycast
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
-                          unsigned long long *stats, int ray_block, hipStream_t s);
+                          unsigned long long *stats, int ray_block, bool o32, hipStream_t s);
 // waves: the waves per SIMD its registers aim for (4 or 5); o32 as for launch_raycast
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tsdf_type, int weight_type,
                           unsigned long long *stats, int ray_block, int waves, bool o32, hipStream_t s);

@@ -597,6 +597,17 @@ struct Vol {
         out = ok ? R::val(r) : 1.0f;
         return ok;
     }
+    // voxel b * 512 + l (brick index b, in-brick offset l < 512); O32 as for
+    // load_at
+    template <bool O32>
+    __device__ __forceinline__ bool load_bl(unsigned b, unsigned l, float &out) const {
+        if (!O32) return load((size_t)b * kBrickVox + l, out);
+        typedef VoxRec<TV, TW> R;
+        const typename R::T r = ld_rec32<TV, TW>(vx, (b << 9) | l);
+        const bool ok = R::has_w(r);
+        out = ok ? R::val(r) : 1.0f;
+        return ok;
+    }
     // d_gtvu, exactly: nearest voxel by ROUNDF; true when it has weight.
     __device__ __forceinline__ bool nearest_valid(f3 v, float &out) const {
         int x, y, z;
@@ -615,6 +626,7 @@ struct Vol {
     // [lo, lo + size)^3 (a superbrick or brick without band bit) reads 1.0,
     // size = -1 when the position is outside the grid (moving towards it),
     // size = 0 otherwise.
+    template <bool O32>
     __device__ __forceinline__ int nearest(f3 v, f3 rd, float &out, int3 &lo, int &size) const {
         out = 1.0f;
         size = 0;
@@ -643,7 +655,7 @@ struct Vol {
             size = (2 * r + 1) * unit;
             return 0;
         }
-        return load((size_t)cb() * kBrickVox + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
+        return load_bl<O32>((unsigned)cb(), (unsigned)(((z & 7) << 6) | ((y & 7) << 3) | (x & 7)), out) ? 1 : 0;
     }
     // ---- instrumented form only: the bytes the reference's own lookups
     // request, tap by tap, in its order and with its early exits.
@@ -695,6 +707,7 @@ struct Vol {
     // 1.0. The 2x2x2 stencil is range-checked as a whole and its 8 values and
     // weights are loaded together; a voxel of an empty brick has weight 0, so
     // no occupancy lookup is needed here.
+    template <bool O32>
     __device__ __forceinline__ bool trilinear(f3 v, float &out) const {
         out = 1.0f;
         if (!finite3(v)) return false;
@@ -723,7 +736,7 @@ struct Vol {
         for (int k = 0; k < 8; ++k) {
             const unsigned b = Bz[k >> 2] + By[(k >> 1) & 1] + Bx[k & 1];
             const unsigned l = Lz[k >> 2] | Ly[(k >> 1) & 1] | Lx[k & 1];
-            ok &= load((size_t)b * kBrickVox + l, f[k]);
+            ok &= load_bl<O32>(b, l, f[k]);
         }
         if (!ok) return false;
         const float fi = (1.0f - ry) * ((1.0f - rx) * f[0] + rx * f[1]) + ry * ((1.0f - rx) * f[2] + rx * f[3]);
@@ -807,7 +820,7 @@ struct RayStats {
     }
 };
 
-template <class TV, class TW, bool STATS>
+template <class TV, class TW, bool STATS, bool O32>
 __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C, const float *Tcg, float px,
                                          float py, f3 &hit, RayStats &rs) {
     const float dist_min = C.dist_cc - C.r_cube, dist_max = C.dist_cc + C.r_cube;
@@ -866,7 +879,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         // instrumented form keeps marching to count the reference's taps.
         int3 lo;
         int size;
-        if (V.nearest(vox, rd, tsdf, lo, size) < 0 && !STATS) return false;
+        if (V.template nearest<O32>(vox, rd, tsdf, lo, size) < 0 && !STATS) return false;
         if (!STATS && size != 0) {
             // 1.0 here and in the region around: skip the steps that stay
             // inside it (tsdf is 1.0, so the next step is the same S each time)
@@ -899,7 +912,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
         }
         if (fabsf(tsdf) < 1.0f) {
             if (STATS) rs.trilinear(V, vox);
-            V.trilinear(vox, tsdf);
+            V.template trilinear<O32>(vox, tsdf);
             if (tsdf < 0.0f) {
                 if (coarse) return false;
                 behind = true;
@@ -916,7 +929,7 @@ __device__ __forceinline__ bool cast_ray(const Vol<TV, TW> &V, const RayConst &C
     float step = C.step_scale * tsdf;
     vox = add(vox, scale(step, rd));
     if (STATS) rs.trilinear(V, vox);
-    V.trilinear(vox, tsdf);
+    V.template trilinear<O32>(vox, tsdf);
     step = C.step_scale * tsdf;
     vox = add(vox, scale(step, rd));
     hit = voxel_to_world(V.P, vox);
@@ -1064,7 +1077,9 @@ __device__ __forceinline__ void ray_pixel(int t, int &px, int &py) {
 // 8 waves per SIMD (at most 64 VGPRs): the march is VALU-bound with dependent
 // taps, and the extra waves hide the taps' latency better than the few
 // registers the compiler would add beyond 64.
-template <class TV, class TW, bool STATS, int RB>
+// O32: the lookups take 32-bit offsets from the voxel-record base (every
+// session of the launch has its records below 4 GB), as in k_raycast_shade.
+template <class TV, class TW, bool STATS, int RB, bool O32>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_raycast(
     const SessParams *__restrict__ SP, LevelWork lw, unsigned long long *stats) {
     __shared__ uint32_t s_dist[kDistCells / 8];
@@ -1102,7 +1117,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     RayStats rs;
     f3 p;
     const M34 Tc = ld_m34(st->T_cg_f);
-    const bool ok = cast_ray<TV, TW, STATS>(V, C, Tc.m, (float)x, (float)y, p, rs);
+    const bool ok = cast_ray<TV, TW, STATS, O32>(V, C, Tc.m, (float)x, (float)y, p, rs);
     const size_t t = (size_t)y * L.W + x;
     if (ok) {
         st_mvtx(P.mvn[level], t, p.x, p.y, p.z);  // world hit; the shading makes it the camera-frame vertex
@@ -1408,17 +1423,28 @@ void launch_integrate(const SessParams *P, int nsess, BrickList bl, int tt, int 
 }
 
 template <class TV, class TW>
-static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, hipStream_t s) {
+static void march_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, bool o32,
+                    hipStream_t s) {
     const int n = lw.blocks();
     if (n <= 0) return;
+#define VS_MARCH(RB_)                                                                                          \
+    do {                                                                                                       \
+        if (o32)                                                                                               \
+            hipLaunchKernelGGL((k_raycast<TV, TW, false, RB_, true>), dim3(n), dim3(kTileThreads), 0, s, P, lw, \
+                               stats);                                                                         \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_raycast<TV, TW, false, RB_, false>), dim3(n), dim3(kTileThreads), 0, s, P, lw, \
+                               stats);                                                                         \
+    } while (0)
     if (stats)
-        hipLaunchKernelGGL((k_raycast<TV, TW, true, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        hipLaunchKernelGGL((k_raycast<TV, TW, true, kRay8x8, false>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
     else if (rb == kRay16x4)
-        hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay16x4>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        VS_MARCH(kRay16x4);
     else if (rb == kRay32x2)
-        hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay32x2>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        VS_MARCH(kRay32x2);
     else
-        hipLaunchKernelGGL((k_raycast<TV, TW, false, kRay8x8>), dim3(n), dim3(kTileThreads), 0, s, P, lw, stats);
+        VS_MARCH(kRay8x8);
+#undef VS_MARCH
 }
 template <class TV, class TW>
 static void shade_t(const SessParams *P, const LevelWork &lw, unsigned long long *stats, int rb, int waves,
@@ -1459,8 +1485,8 @@ LevelWork level_work(const WorkItem *const work[kMaxLevels], const int n[kMaxLev
     return lw;
 }
 void launch_raycast_march(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
-                          int rb, hipStream_t s) {
-    VS_DISPATCH(tt, wt, march_t, P, lw, stats, rb, s);
+                          int rb, bool o32, hipStream_t s) {
+    VS_DISPATCH(tt, wt, march_t, P, lw, stats, rb, o32, s);
 }
 void launch_raycast_shade(const SessParams *P, const LevelWork &lw, int tt, int wt, unsigned long long *stats,
                           int rb, int waves, bool o32, hipStream_t s) {
@@ -1479,13 +1505,13 @@ static LevelWork one_level(const WorkItem *work, int n, int level) {
 void launch_raycast(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt, bool o32,
                     hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
-    launch_raycast_march(P, lw, tt, wt, nullptr, kRay8x8, s);
+    launch_raycast_march(P, lw, tt, wt, nullptr, kRay8x8, o32, s);
     launch_raycast_shade(P, lw, tt, wt, nullptr, kRay8x8, 4, o32, s);
 }
 void launch_raycast_stats(const SessParams *P, const WorkItem *work, int n, int level, int tt, int wt,
                           unsigned long long *stats, hipStream_t s) {
     const LevelWork lw = one_level(work, n, level);
-    launch_raycast_march(P, lw, tt, wt, stats, kRay8x8, s);
+    launch_raycast_march(P, lw, tt, wt, stats, kRay8x8, false, s);
     launch_raycast_shade(P, lw, tt, wt, stats, kRay32x2, 4, false, s);
 }
 
