@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <string>
@@ -998,9 +999,17 @@ vs_status stage_host_frames(HostStage &S, const SessParams *dP, vs_session *cons
         for (size_t j = t; j < jobs.size(); j += nt) memcpy(dst + jobs[j].off, jobs[j].src, jobs[j].bytes);
     };
     if (nt > 1) {
+        // threads that cannot be started leave their share to this one
         std::vector<std::thread> pool;
-        for (int t = 1; t < nt; ++t) pool.emplace_back(copy_range, t);
-        copy_range(0);
+        std::vector<int> mine{0};
+        for (int t = 1; t < nt; ++t) {
+            try {
+                pool.emplace_back(copy_range, t);
+            } catch (const std::exception &) {
+                mine.push_back(t);
+            }
+        }
+        for (int t : mine) copy_range(t);
         for (auto &th : pool) th.join();
     } else {
         copy_range(0);
