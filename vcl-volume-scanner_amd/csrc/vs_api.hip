@@ -1537,9 +1537,10 @@ vs_status vs_fuse_frame(vs_session *s, const uint16_t *depth_mm, int is_host, vs
 namespace {
 // Batches of at least kSplitMin sessions run as kSplitParts parts in a
 // captured step (64 sessions of 640x480 / 512^3: 10.70 ms per step as one
-// chain, 10.37 as 2 parts, 10.15 as 3, 10.21 as 4; with today's kernels
+// chain, 10.37 as 2 parts, 10.15 as 3, 10.21 as 4; with faster kernels
 // 8.30 as 2, 8.15-8.21 as 3, 8.16-8.17 as 4 and 8.74-8.79 as 5, more
-// branches than the 4 hardware queues a process gets).
+// branches than the 4 hardware queues a process gets; later still 6.32 as
+// 2, 6.23 as 3 and 6.27 as 4).
 constexpr int kSplitMin = 16;
 #ifndef VS_SPLIT_PARTS
 #define VS_SPLIT_PARTS 3
